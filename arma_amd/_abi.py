@@ -1,0 +1,334 @@
+"""ctypes binding of libarma_hip.so (the C ABI declared in include/arma_hip.h).
+
+This is the same binding a Python caller of the reference would add around
+its C++ generator and solver; the compute runs in the HIP library, never in
+Python.  Device buffers are torch tensors on a ROCm device (torch is used for
+allocation and streams only).  The library is loaded from the package
+directory; if it is missing, importing fails loudly.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libarma_hip.so")
+
+ARMA_HIP_OK = 0
+ARMA_HIP_EINVAL = -1
+ARMA_HIP_EPRNG = -2
+ARMA_HIP_ERUNTIME = -3
+ARMA_HIP_EDEVICE = -4
+ARMA_HIP_ENOMEM = -5
+
+FP64, FP32 = 0, 1
+NOISE_SEQUENTIAL, NOISE_PARTITIONED = 0, 1
+NOISE_DOUBLE, NOISE_FLOAT = 0, 1
+AR_PLAIN, AR_WAVEFRONT = 0, 1
+
+EXPORTS = [
+    "arma_hip_init", "arma_hip_destroy", "arma_hip_last_error", "arma_hip_set_stream",
+    "arma_hip_get_stream", "arma_hip_synchronize", "arma_hip_white_noise",
+    "arma_hip_ar_recurrence", "arma_hip_ar_generate", "arma_hip_rescale",
+    "arma_hip_linear_potential", "arma_hip_mt_raw", "arma_hip_read_mt_config",
+    "arma_hip_write_mt_config", "arma_hip_dcmt_create", "arma_hip_dcmt_create_many",
+    "arma_hip_acf", "arma_hip_yule_walker_choi", "arma_hip_yule_walker_gauss",
+    "arma_hip_white_noise_variance", "arma_hip_partition_shape",
+]
+
+
+class MtParams(C.Structure):
+    _fields_ = [
+        ("aaa", C.c_uint32), ("mm", C.c_int32), ("nn", C.c_int32), ("rr", C.c_int32),
+        ("ww", C.c_int32), ("wmask", C.c_uint32), ("umask", C.c_uint32),
+        ("lmask", C.c_uint32), ("shift0", C.c_int32), ("shift1", C.c_int32),
+        ("shiftB", C.c_int32), ("shiftC", C.c_int32), ("maskB", C.c_uint32),
+        ("maskC", C.c_uint32),
+    ]
+
+    def as_tuple(self):
+        return tuple(getattr(self, f) for f, _ in self._fields_)
+
+
+class ArDesc(C.Structure):
+    _fields_ = [
+        ("shape", C.c_int32 * 3), ("phi", C.POINTER(C.c_double)), ("phi_shape", C.c_int32 * 3),
+        ("varwn", C.c_double), ("noise_mode", C.c_int32), ("noise_semantics", C.c_int32),
+        ("partition", C.c_int32 * 3), ("mts", C.POINTER(MtParams)), ("n_mts", C.c_int32),
+        ("seeds", C.POINTER(C.c_uint32)), ("precision", C.c_int32), ("algorithm", C.c_int32),
+        ("tile", C.c_int32 * 3),
+    ]
+
+
+class PotDesc(C.Structure):
+    _fields_ = [
+        ("shape", C.c_int32 * 3), ("delta", C.c_double * 3), ("depth", C.c_double),
+        ("lbound", C.c_double * 2), ("ubound", C.c_double * 2), ("npoints", C.c_int32 * 2),
+        ("precision", C.c_int32),
+    ]
+
+
+def mt19937_params():
+    """MT19937 written as a dynamic-creator parameter record."""
+    return MtParams(aaa=0x9908B0DF, mm=397, nn=624, rr=31, ww=32, wmask=0xFFFFFFFF,
+                    umask=0x80000000, lmask=0x7FFFFFFF, shift0=11, shift1=18,
+                    shiftB=7, shiftC=15, maskB=0x9D2C5680, maskC=0xEFC60000)
+
+
+class ArmaHipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{msg} (code {code})")
+        self.code = code
+
+
+class PRNGError(ArmaHipError):
+    """arma::PRNG_error: fewer MT records than streams."""
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.arma_hip_init.argtypes = [C.c_int, C.POINTER(vp)]
+        L.arma_hip_destroy.argtypes = [vp]
+        L.arma_hip_destroy.restype = None
+        L.arma_hip_last_error.restype = C.c_char_p
+        L.arma_hip_set_stream.argtypes = [vp, vp]
+        L.arma_hip_get_stream.argtypes = [vp]
+        L.arma_hip_get_stream.restype = vp
+        L.arma_hip_synchronize.argtypes = [vp]
+        L.arma_hip_white_noise.argtypes = [vp, C.POINTER(ArDesc), vp]
+        L.arma_hip_ar_recurrence.argtypes = [vp, C.POINTER(ArDesc), vp]
+        L.arma_hip_ar_generate.argtypes = [vp, C.POINTER(ArDesc), vp, C.c_int]
+        L.arma_hip_rescale.argtypes = [vp, vp, C.POINTER(C.c_int32), C.c_int32, C.c_double,
+                                       C.POINTER(C.c_double)]
+        L.arma_hip_linear_potential.argtypes = [vp, C.POINTER(PotDesc), vp, vp, C.c_int, vp]
+        L.arma_hip_mt_raw.argtypes = [vp, C.POINTER(MtParams), vp, C.c_int32, C.c_int32, vp]
+        L.arma_hip_read_mt_config.argtypes = [C.c_char_p, C.c_int32, C.POINTER(MtParams),
+                                              C.POINTER(C.c_int32)]
+        L.arma_hip_write_mt_config.argtypes = [C.c_char_p, C.POINTER(MtParams), C.c_int32]
+        L.arma_hip_dcmt_create.argtypes = [C.c_int32, C.c_int32, C.c_uint32, C.POINTER(MtParams)]
+        L.arma_hip_dcmt_create_many.argtypes = [C.c_int32, C.c_int32, C.c_uint32, C.c_int32,
+                                                C.POINTER(MtParams)]
+        dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+        i3 = C.c_int32 * 3
+        L.arma_hip_acf.argtypes = [C.c_int32, C.c_double * 3, i3, C.c_double, C.c_double,
+                                   C.c_double * 3, C.c_double * 2, dp]
+        L.arma_hip_yule_walker_choi.argtypes = [dp, C.c_int32, C.c_int32, C.c_double, C.c_int32,
+                                                C.c_int32, C.c_double, dp, i3,
+                                                C.POINTER(C.c_double)]
+        L.arma_hip_yule_walker_gauss.argtypes = [dp, i3, i3, dp]
+        L.arma_hip_white_noise_variance.argtypes = [dp, i3, dp, i3]
+        L.arma_hip_white_noise_variance.restype = C.c_double
+        L.arma_hip_partition_shape.argtypes = [i3, i3, i3, C.c_int32, i3]
+        L.arma_hip_partition_shape.restype = None
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != ARMA_HIP_OK:
+        msg = lib().arma_hip_last_error().decode(errors="replace")
+        if rc == ARMA_HIP_EPRNG:
+            raise PRNGError(rc, msg)
+        if rc == ARMA_HIP_EINVAL:
+            raise ValueError(f"{msg} (code {rc})")
+        raise ArmaHipError(rc, msg)
+
+
+def params_array(plist):
+    arr = (MtParams * len(plist))()
+    for i, p in enumerate(plist):
+        arr[i] = p
+    return arr
+
+
+def dcmt_create(p, id_, seed):
+    out = MtParams()
+    check(lib().arma_hip_dcmt_create(p, id_, seed, C.byref(out)))
+    return out
+
+
+def dcmt_create_many(p, n, seed, nthreads=None):
+    if nthreads is None:
+        nthreads = min(16, os.cpu_count() or 1)
+    arr = (MtParams * n)()
+    check(lib().arma_hip_dcmt_create_many(p, n, seed, nthreads, arr))
+    return arr
+
+
+def read_mt_config(path, n):
+    arr = (MtParams * n)()
+    got = C.c_int32()
+    check(lib().arma_hip_read_mt_config(path.encode(), n, arr, C.byref(got)))
+    return arr
+
+
+def write_mt_config(path, arr):
+    check(lib().arma_hip_write_mt_config(path.encode(), arr, len(arr)))
+
+
+ACF_STANDING_WAVE, ACF_PROPAGATING_WAVE = 0, 1
+
+
+def acf(func, shape, delta=(1.0, 1.0, 1.0), amplitude=1.0, velocity=1.0,
+        alpha=(0.06, 0.06, 0.06), beta=(0.8, 0.0)):
+    """Closed-form ACF (defaults of src/bits/acf_wrapper.hh:23-28)."""
+    out = np.empty(tuple(int(s) for s in shape), np.float64)
+    check(lib().arma_hip_acf(func, (C.c_double * 3)(*delta), (C.c_int32 * 3)(*shape),
+                             amplitude, velocity, (C.c_double * 3)(*alpha),
+                             (C.c_double * 2)(*beta), out.reshape(-1)))
+    return out
+
+
+def yule_walker_choi(acf_cube, max_order, var_eps=1e-6, determine_order=True, chop=True,
+                     chop_eps=1e-10):
+    """Choi coefficients; the AR model calls this with var_eps = 1e-6
+    (src/generator/ar_model.cc:142-150).  Returns (phi, solver variance)."""
+    a = np.ascontiguousarray(acf_cube, np.float64)
+    n = a.shape[0]
+    phi = np.zeros(max(max_order, 2) ** 3 + n ** 3, np.float64)
+    order = (C.c_int32 * 3)()
+    var = C.c_double()
+    check(lib().arma_hip_yule_walker_choi(a.reshape(-1), n, max_order, var_eps,
+                                          int(determine_order), int(chop), chop_eps, phi, order,
+                                          C.byref(var)))
+    shp = tuple(order)
+    return phi[: int(np.prod(shp))].reshape(shp).copy(), var.value
+
+
+def yule_walker_gauss(acf_cube, order):
+    a = np.ascontiguousarray(acf_cube, np.float64)
+    phi = np.zeros(int(np.prod(order)), np.float64)
+    check(lib().arma_hip_yule_walker_gauss(a.reshape(-1), (C.c_int32 * 3)(*a.shape),
+                                           (C.c_int32 * 3)(*order), phi))
+    return phi.reshape(tuple(order))
+
+
+def white_noise_variance(acf_cube, phi):
+    a = np.ascontiguousarray(acf_cube, np.float64)
+    p = np.ascontiguousarray(phi, np.float64)
+    return lib().arma_hip_white_noise_variance(a.reshape(-1), (C.c_int32 * 3)(*a.shape),
+                                               p.reshape(-1), (C.c_int32 * 3)(*p.shape))
+
+
+def partition_shape(partition, grid, order, parallelism):
+    out = (C.c_int32 * 3)()
+    lib().arma_hip_partition_shape((C.c_int32 * 3)(*partition), (C.c_int32 * 3)(*grid),
+                                   (C.c_int32 * 3)(*order), parallelism, out)
+    return tuple(out)
+
+
+def _ptr(t):
+    """Device or host address of a torch tensor / numpy array."""
+    if isinstance(t, np.ndarray):
+        return t.ctypes.data
+    return t.data_ptr()
+
+
+class Context:
+    """One arma_hip context: a device, a HIP stream and the device buffers."""
+
+    def __init__(self, device=0, torch_stream=True):
+        self.h = C.c_void_p()
+        check(lib().arma_hip_init(device, C.byref(self.h)))
+        self.device = device
+        if torch_stream:
+            # order the library's kernels with torch's work on this device
+            import torch
+            with torch.cuda.device(device):
+                self.set_stream(torch.cuda.current_stream().cuda_stream)
+
+    def close(self):
+        if self.h:
+            lib().arma_hip_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, stream_handle):
+        check(lib().arma_hip_set_stream(self.h, C.c_void_p(stream_handle)))
+
+    def synchronize(self):
+        check(lib().arma_hip_synchronize(self.h))
+
+    @staticmethod
+    def ar_desc(shape, phi, varwn, mts, seeds=None, noise_mode=NOISE_SEQUENTIAL,
+                partition=(0, 0, 0), precision=FP64, algorithm=AR_PLAIN, tile=(0, 0, 0),
+                noise_semantics=NOISE_DOUBLE):
+        phi = np.ascontiguousarray(phi, np.float64)
+        d = ArDesc()
+        d.shape[:] = [int(s) for s in shape]
+        d.phi = phi.ctypes.data_as(C.POINTER(C.c_double))
+        d.phi_shape[:] = [int(s) for s in phi.shape]
+        d.varwn = float(varwn)
+        d.noise_mode = noise_mode
+        d.noise_semantics = noise_semantics
+        d.partition[:] = [int(s) for s in partition]
+        marr = mts if isinstance(mts, C.Array) else params_array(list(mts))
+        d.mts = C.cast(marr, C.POINTER(MtParams))
+        d.n_mts = len(marr)
+        sarr = None
+        if seeds is not None:
+            sarr = np.ascontiguousarray(seeds, np.uint32)
+            d.seeds = sarr.ctypes.data_as(C.POINTER(C.c_uint32))
+        d.precision = precision
+        d.algorithm = algorithm
+        d.tile[:] = [int(s) for s in tile]
+        d._keep = (phi, marr, sarr)  # keep host buffers alive with the descriptor
+        return d
+
+    def white_noise(self, desc, eps):
+        check(lib().arma_hip_white_noise(self.h, C.byref(desc), C.c_void_p(_ptr(eps))))
+
+    def ar_recurrence(self, desc, zeta):
+        check(lib().arma_hip_ar_recurrence(self.h, C.byref(desc), C.c_void_p(_ptr(zeta))))
+
+    def ar_generate(self, desc, zeta, on_device=True):
+        check(lib().arma_hip_ar_generate(self.h, C.byref(desc), C.c_void_p(_ptr(zeta)),
+                                         1 if on_device else 0))
+
+    def rescale(self, zeta, shape, precision, gamma0):
+        f = C.c_double()
+        s = (C.c_int32 * 3)(*[int(v) for v in shape])
+        check(lib().arma_hip_rescale(self.h, C.c_void_p(_ptr(zeta)), s, precision, gamma0,
+                                     C.byref(f)))
+        return f.value
+
+    @staticmethod
+    def pot_desc(shape, delta, depth, lbound, ubound, npoints, precision=FP64):
+        d = PotDesc()
+        d.shape[:] = [int(s) for s in shape]
+        d.delta[:] = [float(v) for v in delta]
+        d.depth = float(depth)
+        d.lbound[:] = [float(v) for v in lbound]
+        d.ubound[:] = [float(v) for v in ubound]
+        d.npoints[:] = [int(v) for v in npoints]
+        d.precision = precision
+        return d
+
+    def linear_potential(self, desc, zeta, phi, on_device=True, want_wnmax=False):
+        wn = None
+        if want_wnmax:
+            wn = np.empty(2 * desc.npoints[0], np.float64)
+        check(lib().arma_hip_linear_potential(self.h, C.byref(desc), C.c_void_p(_ptr(zeta)),
+                                              C.c_void_p(_ptr(phi)), 1 if on_device else 0,
+                                              None if wn is None else wn.ctypes.data))
+        return None if wn is None else wn.reshape(-1, 2)
+
+    def mt_raw(self, mts, seeds, draws, out):
+        marr = mts if isinstance(mts, C.Array) else params_array(list(mts))
+        sarr = None if seeds is None else np.ascontiguousarray(seeds, np.uint32)
+        check(lib().arma_hip_mt_raw(self.h, C.cast(marr, C.POINTER(MtParams)),
+                                    None if sarr is None else sarr.ctypes.data, len(marr), draws,
+                                    C.c_void_p(_ptr(out))))

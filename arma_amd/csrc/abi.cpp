@@ -1,0 +1,326 @@
+// C ABI of libarma_hip: context management, argument validation, host/device
+// staging, mt.dat input/output.  Every entry point converts exceptions into
+// the ARMA_HIP_* codes and keeps the message for arma_hip_last_error().
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <class F>
+int guarded(F&& f) {
+	try {
+		f();
+		return ARMA_HIP_OK;
+	} catch (const arma_hip::Error& e) {
+		g_last_error = e.what();
+		return e.code;
+	} catch (const std::bad_alloc&) {
+		g_last_error = "out of host memory";
+		return ARMA_HIP_ENOMEM;
+	} catch (const std::exception& e) {
+		g_last_error = e.what();
+		return ARMA_HIP_ERUNTIME;
+	}
+}
+
+void use_device(arma_hip_ctx* ctx) {
+	ARMA_HIP_REQUIRE(ctx != nullptr, ARMA_HIP_EINVAL, "null context");
+	ARMA_HIP_CHECK(hipSetDevice(ctx->device));
+}
+
+size_t elem_size(int precision) { return precision == ARMA_FP32 ? 4 : 8; }
+
+void check_ar_desc(const arma_ar_desc* d) {
+	ARMA_HIP_REQUIRE(d != nullptr, ARMA_HIP_EINVAL, "null descriptor");
+	for (int k = 0; k < 3; ++k) {
+		ARMA_HIP_REQUIRE(d->shape[k] > 0, ARMA_HIP_EINVAL, "bad output grid shape");
+		ARMA_HIP_REQUIRE(d->phi_shape[k] > 0 && d->phi_shape[k] <= 64, ARMA_HIP_EINVAL,
+		                 "bad AR order");
+	}
+	ARMA_HIP_REQUIRE(d->phi != nullptr, ARMA_HIP_EINVAL, "null AR coefficients");
+	ARMA_HIP_REQUIRE(d->precision == ARMA_FP64 || d->precision == ARMA_FP32, ARMA_HIP_EINVAL,
+	                 "bad precision");
+	ARMA_HIP_REQUIRE(arma_hip::numel3(d->shape) < (size_t(1) << 40), ARMA_HIP_EINVAL,
+	                 "output grid too large");
+	for (size_t i = 0; i < arma_hip::numel3(d->phi_shape); ++i)
+		ARMA_HIP_REQUIRE(std::isfinite(d->phi[i]), ARMA_HIP_EINVAL, "non-finite AR coefficient");
+}
+
+void recurrence(arma_hip_ctx* ctx, const arma_ar_desc* d, void* zeta) {
+	if (d->algorithm == ARMA_AR_PLAIN) {
+		arma_hip::launch_ar_plain(ctx, d, zeta);
+	} else if (d->algorithm == ARMA_AR_WAVEFRONT) {
+		arma_hip::launch_ar_wavefront(ctx, d, zeta);
+	} else {
+		throw arma_hip::Error(ARMA_HIP_EINVAL, "bad AR algorithm");
+	}
+}
+
+// mt.dat record: the raw mt_struct (15 four-byte fields, 4 bytes of padding
+// and the 8-byte state pointer on LP64) followed by nn state words.
+constexpr size_t kRecordHeader = 72;
+
+}  // namespace
+
+extern "C" {
+
+const char* arma_hip_last_error(void) { return g_last_error.c_str(); }
+
+int arma_hip_init(int device, arma_hip_ctx** out) {
+	return guarded([&] {
+		ARMA_HIP_REQUIRE(out != nullptr, ARMA_HIP_EINVAL, "null output pointer");
+		int n = 0;
+		ARMA_HIP_CHECK(hipGetDeviceCount(&n));
+		ARMA_HIP_REQUIRE(device >= 0 && device < n, ARMA_HIP_EDEVICE, "no such HIP device");
+		ARMA_HIP_CHECK(hipSetDevice(device));
+		auto* ctx = new arma_hip_ctx;
+		ctx->device = device;
+		hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+		if (e != hipSuccess) {
+			delete ctx;
+			throw arma_hip::Error(ARMA_HIP_EDEVICE, hipGetErrorString(e));
+		}
+		ctx->own_stream = true;
+		(void)hipDeviceGetAttribute(&ctx->num_cus, hipDeviceAttributeMultiprocessorCount, device);
+		*out = ctx;
+	});
+}
+
+void arma_hip_destroy(arma_hip_ctx* ctx) {
+	if (!ctx) return;
+	(void)hipSetDevice(ctx->device);
+	(void)hipStreamSynchronize(ctx->stream);
+	for (arma_hip::Scratch* s : {&ctx->params, &ctx->seeds, &ctx->state, &ctx->phi,
+	                             &ctx->partials, &ctx->flags, &ctx->noise_tmp, &ctx->pot_spec,
+	                             &ctx->pot_mid, &ctx->pot_wn, &ctx->pot_twiddle, &ctx->pot_kmin,
+	                             &ctx->zeta_tmp, &ctx->phi_tmp})
+		s->release();
+	if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
+	delete ctx;
+}
+
+int arma_hip_set_stream(arma_hip_ctx* ctx, void* stream) {
+	return guarded([&] {
+		use_device(ctx);
+		ARMA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+		if (ctx->own_stream) ARMA_HIP_CHECK(hipStreamDestroy(ctx->stream));
+		ctx->stream = (hipStream_t)stream;
+		ctx->own_stream = false;
+	});
+}
+
+void* arma_hip_get_stream(arma_hip_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int arma_hip_synchronize(arma_hip_ctx* ctx) {
+	return guarded([&] {
+		use_device(ctx);
+		ARMA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+	});
+}
+
+int arma_hip_white_noise(arma_hip_ctx* ctx, const arma_ar_desc* desc, void* eps) {
+	return guarded([&] {
+		use_device(ctx);
+		check_ar_desc(desc);
+		ARMA_HIP_REQUIRE(eps != nullptr, ARMA_HIP_EINVAL, "null output buffer");
+		arma_hip::launch_white_noise(ctx, desc, eps);
+	});
+}
+
+int arma_hip_ar_recurrence(arma_hip_ctx* ctx, const arma_ar_desc* desc, void* zeta) {
+	return guarded([&] {
+		use_device(ctx);
+		check_ar_desc(desc);
+		ARMA_HIP_REQUIRE(zeta != nullptr, ARMA_HIP_EINVAL, "null surface buffer");
+		recurrence(ctx, desc, zeta);
+	});
+}
+
+int arma_hip_ar_generate(arma_hip_ctx* ctx, const arma_ar_desc* desc, void* zeta,
+                         int zeta_on_device) {
+	return guarded([&] {
+		use_device(ctx);
+		check_ar_desc(desc);
+		ARMA_HIP_REQUIRE(zeta != nullptr, ARMA_HIP_EINVAL, "null surface buffer");
+		const size_t bytes = arma_hip::numel3(desc->shape) * elem_size(desc->precision);
+		void* dz = zeta;
+		if (!zeta_on_device) dz = ctx->zeta_tmp.get(bytes);
+		arma_hip::launch_white_noise(ctx, desc, dz);
+		recurrence(ctx, desc, dz);
+		if (!zeta_on_device) {
+			ARMA_HIP_CHECK(hipMemcpyAsync(zeta, dz, bytes, hipMemcpyDeviceToHost, ctx->stream));
+			ARMA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+		}
+	});
+}
+
+int arma_hip_rescale(arma_hip_ctx* ctx, void* zeta, const int32_t shape[3], int32_t precision,
+                     double gamma0, double* factor_out) {
+	return guarded([&] {
+		use_device(ctx);
+		ARMA_HIP_REQUIRE(zeta && shape, ARMA_HIP_EINVAL, "null argument");
+		ARMA_HIP_REQUIRE(shape[0] > 0 && shape[1] > 0 && shape[2] > 0, ARMA_HIP_EINVAL,
+		                 "bad shape");
+		ARMA_HIP_REQUIRE(precision == ARMA_FP64 || precision == ARMA_FP32, ARMA_HIP_EINVAL,
+		                 "bad precision");
+		const double f = arma_hip::launch_rescale(ctx, zeta, shape, precision, gamma0);
+		if (factor_out) *factor_out = f;
+	});
+}
+
+int arma_hip_linear_potential(arma_hip_ctx* ctx, const arma_pot_desc* desc, const void* zeta,
+                              void* phi, int on_device, double* wnmax_out) {
+	return guarded([&] {
+		use_device(ctx);
+		ARMA_HIP_REQUIRE(desc && zeta && phi, ARMA_HIP_EINVAL, "null argument");
+		for (int k = 0; k < 3; ++k)
+			ARMA_HIP_REQUIRE(desc->shape[k] > 0, ARMA_HIP_EINVAL, "bad surface shape");
+		ARMA_HIP_REQUIRE(desc->npoints[0] > 0 && desc->npoints[1] > 0, ARMA_HIP_EINVAL,
+		                 "bad domain");
+		ARMA_HIP_REQUIRE(desc->precision == ARMA_FP64 || desc->precision == ARMA_FP32,
+		                 ARMA_HIP_EINVAL, "bad precision");
+		// src/velocity/basic_solver.cc:166-169
+		ARMA_HIP_REQUIRE(!(desc->lbound[1] < -desc->depth), ARMA_HIP_EINVAL,
+		                 "min(z) is less than -h");
+		const size_t es = elem_size(desc->precision);
+		const size_t zbytes = arma_hip::numel3(desc->shape) * es;
+		const size_t pbytes = size_t(desc->npoints[0]) * desc->npoints[1] * desc->shape[1] *
+		                      desc->shape[2] * es;
+		const void* dz = zeta;
+		void* dp = phi;
+		if (!on_device) {
+			void* tz = ctx->zeta_tmp.get(zbytes);
+			ARMA_HIP_CHECK(hipMemcpyAsync(tz, zeta, zbytes, hipMemcpyHostToDevice, ctx->stream));
+			dz = tz;
+			dp = ctx->phi_tmp.get(pbytes);
+		}
+		arma_hip::launch_linear_potential(ctx, desc, dz, dp, wnmax_out);
+		if (!on_device) {
+			ARMA_HIP_CHECK(hipMemcpyAsync(phi, dp, pbytes, hipMemcpyDeviceToHost, ctx->stream));
+			ARMA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+		}
+	});
+}
+
+int arma_hip_profile(arma_hip_ctx* ctx, int32_t enable) {
+	return guarded([&] {
+		use_device(ctx);
+		ctx->profiling = enable != 0;
+	});
+}
+
+int arma_hip_profile_report(arma_hip_ctx* ctx, char* buf, size_t len, int32_t reset) {
+	return guarded([&] {
+		use_device(ctx);
+		for (auto& r : ctx->pending) {
+			ARMA_HIP_CHECK(hipEventSynchronize(r.stop));
+			float ms = 0.0f;
+			ARMA_HIP_CHECK(hipEventElapsedTime(&ms, r.start, r.stop));
+			auto& tot = ctx->totals[r.name];
+			tot.ms += ms;
+			tot.count += 1;
+			(void)hipEventDestroy(r.start);
+			(void)hipEventDestroy(r.stop);
+		}
+		ctx->pending.clear();
+		std::string out;
+		for (const auto& kv : ctx->totals) {
+			char line[256];
+			std::snprintf(line, sizeof line, "prfl %s %.3fus %lld\n", kv.first.c_str(),
+			              kv.second.ms * 1000.0, kv.second.count);
+			out += line;
+		}
+		if (buf && len) {
+			std::strncpy(buf, out.c_str(), len - 1);
+			buf[len - 1] = 0;
+		}
+		if (reset) ctx->totals.clear();
+	});
+}
+
+int arma_hip_mt_raw(arma_hip_ctx* ctx, const arma_mt_params* mts, const uint32_t* seeds,
+                    int32_t n_streams, int32_t draws_per_stream, uint32_t* out) {
+	return guarded([&] {
+		use_device(ctx);
+		ARMA_HIP_REQUIRE(out != nullptr, ARMA_HIP_EINVAL, "null output");
+		arma_hip::launch_mt_raw(ctx, mts, seeds, n_streams, draws_per_stream, out);
+	});
+}
+
+int arma_hip_read_mt_config(const char* path, int32_t n, arma_mt_params* out, int32_t* n_read) {
+	return guarded([&] {
+		ARMA_HIP_REQUIRE(path && out && n >= 0, ARMA_HIP_EINVAL, "null argument");
+		std::ifstream in(path, std::ios::binary);
+		if (!in.is_open()) {
+			throw arma_hip::Error(ARMA_HIP_ERUNTIME,
+			                      std::string("Mersenne Twister configuration file not found: ") + path);
+		}
+		int32_t k = 0;
+		unsigned char hdr[kRecordHeader];
+		while (k < n && in.read((char*)hdr, kRecordHeader)) {
+			uint32_t f[15];
+			std::memcpy(f, hdr, sizeof f);
+			arma_mt_params q;
+			q.aaa = f[0];
+			q.mm = int32_t(f[1]);
+			q.nn = int32_t(f[2]);
+			q.rr = int32_t(f[3]);
+			q.ww = int32_t(f[4]);
+			q.wmask = f[5];
+			q.umask = f[6];
+			q.lmask = f[7];
+			q.shift0 = int32_t(f[8]);
+			q.shift1 = int32_t(f[9]);
+			q.shiftB = int32_t(f[10]);
+			q.shiftC = int32_t(f[11]);
+			q.maskB = f[12];
+			q.maskC = f[13];
+			ARMA_HIP_REQUIRE(q.nn > 0 && q.nn <= 1392, ARMA_HIP_ERUNTIME, "corrupt mt.dat record");
+			in.ignore(std::streamsize(q.nn) * 4);
+			if (!in) break;
+			out[k++] = q;
+		}
+		if (n_read) *n_read = k;
+		if (k < n) throw arma_hip::Error(ARMA_HIP_EPRNG, "bad number of MT configs");
+	});
+}
+
+int arma_hip_write_mt_config(const char* path, const arma_mt_params* params, int32_t n) {
+	return guarded([&] {
+		ARMA_HIP_REQUIRE(path && params && n >= 0, ARMA_HIP_EINVAL, "null argument");
+		std::ofstream out(path, std::ios::binary);
+		ARMA_HIP_REQUIRE(out.is_open(), ARMA_HIP_ERUNTIME, "cannot open mt.dat for writing");
+		for (int32_t k = 0; k < n; ++k) {
+			const arma_mt_params& q = params[k];
+			unsigned char hdr[kRecordHeader];
+			std::memset(hdr, 0, sizeof hdr);
+			const uint32_t f[15] = {q.aaa, uint32_t(q.mm), uint32_t(q.nn), uint32_t(q.rr),
+			                        uint32_t(q.ww), q.wmask, q.umask, q.lmask,
+			                        uint32_t(q.shift0), uint32_t(q.shift1), uint32_t(q.shiftB),
+			                        uint32_t(q.shiftC), q.maskB, q.maskC, uint32_t(q.nn)};
+			std::memcpy(hdr, f, sizeof f);
+			out.write((const char*)hdr, kRecordHeader);
+			// state words as sgenrand_mt(0) leaves them; readers re-seed anyway
+			uint32_t s = 0;
+			for (int i = 0; i < q.nn; ++i) {
+				const uint32_t w = s & q.wmask;
+				out.write((const char*)&w, 4);
+				s = 1812433253u * (s ^ (s >> 30)) + uint32_t(i + 1);
+			}
+		}
+		ARMA_HIP_REQUIRE(bool(out), ARMA_HIP_ERUNTIME, "short write to mt.dat");
+	});
+}
+
+}  // extern "C"

@@ -1,0 +1,435 @@
+// White noise on the GPU: dynamic-creator Mersenne Twister streams feeding
+// the libstdc++ polar normal transform.
+//
+// Reference semantics:
+//   genrand_mt / sgenrand_mt ........ src/parallel_mt.hh:217-262 (libdcmt)
+//   generate_canonical, polar normal  /usr/include/c++/11/bits/random.tcc:
+//                                     3348-3380, 1802-1836
+//   whole-grid noise (framework=none) src/white_noise.hh:23-58,
+//                                     src/generator/basic_arma_model.cc:36-48
+//   per-partition noise ............. src/generator/ar_model_wn.cc:3-18 with
+//                                     partitions of ar_model_parallel.cc:51-72
+//
+// Every stream is sequential by definition (the polar method rejects a
+// data-dependent number of draws), so the partitioned kernel runs one
+// stream per thread with the generator state in LDS, interleaved across the
+// workgroup so that every state access is bank-conflict free.  The
+// whole-grid (single stream) mode runs one workgroup that twists the state
+// in parallel chunks, evaluates 256 polar attempts at a time and compacts
+// the accepted ones with a workgroup prefix sum.
+//
+// All arithmetic is compiled without FP contraction, so every draw is
+// computed with the roundings the reference source writes.
+#include "common.hpp"
+
+#include <vector>
+
+namespace arma_hip {
+namespace {
+
+constexpr int kMaxNN = 1392;  // largest dcmt state (p = 44497)
+
+__device__ __forceinline__ uint32_t mt_temper(uint32_t x, const arma_mt_params& p) {
+	x ^= x >> p.shift0;
+	x ^= (x << p.shiftB) & p.maskB;
+	x ^= (x << p.shiftC) & p.maskC;
+	x ^= x >> p.shift1;
+	return x;
+}
+
+__device__ __forceinline__ uint32_t mt_twist_word(uint32_t hi, uint32_t lo, uint32_t src,
+                                                  const arma_mt_params& p) {
+	const uint32_t x = (hi & p.umask) | (lo & p.lmask);
+	return src ^ (x >> 1) ^ ((x & 1u) ? p.aaa : 0u);
+}
+
+// Generator whose nn state words live in LDS at st[k * stride].
+struct LdsMt {
+	uint32_t* st;
+	int stride;
+	int i;
+	arma_mt_params p;
+
+	__device__ __forceinline__ uint32_t get(int k) const { return st[k * stride]; }
+	__device__ __forceinline__ void set(int k, uint32_t v) { st[k * stride] = v; }
+
+	__device__ void seed(uint32_t s) {
+		for (int k = 0; k < p.nn; ++k) {
+			set(k, s & p.wmask);
+			s = 1812433253u * (s ^ (s >> 30)) + uint32_t(k + 1);
+		}
+		i = p.nn;
+	}
+
+	__device__ uint32_t next() {
+		if (i >= p.nn) {
+			const int n = p.nn, m = p.mm;
+			int k = 0;
+			for (; k < n - m; ++k) set(k, mt_twist_word(get(k), get(k + 1), get(k + m), p));
+			for (; k < n - 1; ++k) set(k, mt_twist_word(get(k), get(k + 1), get(k + m - n), p));
+			set(n - 1, mt_twist_word(get(n - 1), get(0), get(m - 1), p));
+			i = 0;
+		}
+		return mt_temper(get(i++), p);
+	}
+};
+
+// generate_canonical<double,53> over a 32-bit engine: two draws.
+__device__ __forceinline__ double canonical_from(uint32_t u0, uint32_t u1) {
+	double sum = double(u0);
+	sum = sum + double(u1) * 4294967296.0;
+	double ret = sum / 18446744073709551616.0;
+	if (ret >= 1.0) ret = 0x1.fffffffffffffp-1;  // nextafter(1, 0)
+	return ret;
+}
+
+// generate_canonical<float,24>: one draw.
+__device__ __forceinline__ float canonical_from_f(uint32_t u0) {
+	float sum = float(u0);
+	float ret = sum / 4294967296.0f;
+	if (ret >= 1.0f) ret = 0x1.fffffep-1f;
+	return ret;
+}
+
+template <class G>
+__device__ __forceinline__ double canonical_d(G& g) {
+	const uint32_t u0 = g.next();
+	const uint32_t u1 = g.next();
+	return canonical_from(u0, u1);
+}
+
+// Polar normal generator with the libstdc++ cache of the second variate.
+struct PolarD {
+	bool saved_available = false;
+	double saved = 0.0;
+
+	template <class G>
+	__device__ double next(G& g, double stddev) {
+		double ret;
+		if (saved_available) {
+			saved_available = false;
+			ret = saved;
+		} else {
+			double x, y, r2;
+			do {
+				x = 2.0 * canonical_d(g) - 1.0;
+				y = 2.0 * canonical_d(g) - 1.0;
+				r2 = x * x + y * y;
+			} while (r2 > 1.0 || r2 == 0.0);
+			const double mult = sqrt(-2.0 * log(r2) / r2);
+			saved = x * mult;
+			saved_available = true;
+			ret = y * mult;
+		}
+		return ret * stddev + 0.0;
+	}
+};
+
+struct PolarF {
+	bool saved_available = false;
+	float saved = 0.0f;
+
+	template <class G>
+	__device__ float next(G& g, float stddev) {
+		float ret;
+		if (saved_available) {
+			saved_available = false;
+			ret = saved;
+		} else {
+			float x, y, r2;
+			do {
+				x = 2.0f * canonical_from_f(g.next()) - 1.0f;
+				y = 2.0f * canonical_from_f(g.next()) - 1.0f;
+				r2 = x * x + y * y;
+			} while (r2 > 1.0f || r2 == 0.0f);
+			const float mult = sqrtf(-2.0f * logf(r2) / r2);
+			saved = x * mult;
+			saved_available = true;
+			ret = y * mult;
+		}
+		return ret * stddev + 0.0f;
+	}
+};
+
+// One thread per partition; partitions in row-major order (stream p fills
+// partition p), noise row-major inside each partition.
+template <class OutT, int SEM>
+__global__ void noise_partitioned_kernel(const arma_mt_params* __restrict__ params,
+                                         const uint32_t* __restrict__ seeds, int3 shape,
+                                         int3 part, int3 np, double stddev, float stddev_f,
+                                         OutT* __restrict__ out) {
+	extern __shared__ uint32_t lds_state[];
+	const int tid = threadIdx.x;
+	const int p = blockIdx.x * blockDim.x + tid;
+	const int ntotal = np.x * np.y * np.z;
+	if (p >= ntotal) return;
+	LdsMt g;
+	g.st = lds_state + tid;
+	g.stride = blockDim.x;
+	g.p = params[p];
+	g.seed(seeds ? seeds[p] : 0u);
+	const int a = p / (np.y * np.z), b = (p / np.z) % np.y, c = p % np.z;
+	const int lo0 = a * part.x, lo1 = b * part.y, lo2 = c * part.z;
+	const int e0 = min(lo0 + part.x, shape.x) - lo0;
+	const int e1 = min(lo1 + part.y, shape.y) - lo1;
+	const int e2 = min(lo2 + part.z, shape.z) - lo2;
+	PolarD nd;
+	PolarF nf;
+	for (int t = 0; t < e0; ++t) {
+		for (int x = 0; x < e1; ++x) {
+			OutT* row = out + ((size_t)(lo0 + t) * shape.y + (lo1 + x)) * shape.z + lo2;
+			for (int y = 0; y < e2; ++y) {
+				if (SEM == ARMA_NOISE_DOUBLE) {
+					row[y] = OutT(nd.next(g, stddev));
+				} else {
+					row[y] = OutT(nf.next(g, stddev_f));
+				}
+			}
+		}
+	}
+}
+
+// Raw tempered draws, one thread per stream.
+__global__ void mt_raw_kernel(const arma_mt_params* __restrict__ params,
+                              const uint32_t* __restrict__ seeds, int n_streams, int draws,
+                              uint32_t* __restrict__ out) {
+	extern __shared__ uint32_t lds_state[];
+	const int tid = threadIdx.x;
+	const int s = blockIdx.x * blockDim.x + tid;
+	if (s >= n_streams) return;
+	LdsMt g;
+	g.st = lds_state + tid;
+	g.stride = blockDim.x;
+	g.p = params[s];
+	g.seed(seeds ? seeds[s] : 0u);
+	for (int k = 0; k < draws; ++k) out[(size_t)s * draws + k] = g.next();
+}
+
+// ---- single stream over the whole grid, one cooperative workgroup ----
+
+constexpr int kSeqThreads = 256;
+constexpr int kSeqBuf = 4096 + kMaxNN;
+
+// Chunked parallel twist: within a chunk of c = nn - mm words every new word
+// depends only on old words and on words of earlier chunks.
+__device__ void twist_parallel(uint32_t* st, const arma_mt_params& p, int tid) {
+	const int n = p.nn, m = p.mm, c = n - m;
+	for (int k0 = 0; k0 < n; k0 += c) {
+		const int k1 = min(k0 + c, n);
+		uint32_t v[8];
+#pragma unroll
+		for (int j = 0; j < 8; ++j) {
+			const int k = k0 + tid + j * kSeqThreads;
+			if (k < k1) {
+				if (k < n - m) v[j] = mt_twist_word(st[k], st[k + 1], st[k + m], p);
+				else if (k < n - 1) v[j] = mt_twist_word(st[k], st[k + 1], st[k + m - n], p);
+				else v[j] = mt_twist_word(st[n - 1], st[0], st[m - 1], p);
+			}
+		}
+		__syncthreads();
+#pragma unroll
+		for (int j = 0; j < 8; ++j) {
+			const int k = k0 + tid + j * kSeqThreads;
+			if (k < k1) st[k] = v[j];
+		}
+		__syncthreads();
+	}
+}
+
+template <class OutT, int SEM>
+__global__ __launch_bounds__(kSeqThreads) void noise_sequential_kernel(
+	arma_mt_params p, uint32_t seed, double stddev, float stddev_f, OutT* __restrict__ out,
+	long long n) {
+	__shared__ uint32_t st[kMaxNN];
+	__shared__ uint32_t buf[kSeqBuf];
+	__shared__ int wave_tot[kSeqThreads / 64];
+	const int tid = threadIdx.x;
+	const int lane = tid & 63, wave = tid >> 6;
+	if (tid == 0) {
+		uint32_t s = seed;
+		for (int k = 0; k < p.nn; ++k) {
+			st[k] = s & p.wmask;
+			s = 1812433253u * (s ^ (s >> 30)) + uint32_t(k + 1);
+		}
+	}
+	__syncthreads();
+	constexpr int words_per_attempt = SEM == ARMA_NOISE_DOUBLE ? 4 : 2;
+	constexpr int round_words = kSeqThreads * words_per_attempt;
+	int avail = 0;
+	long long produced = 0;
+	while (produced < n) {
+		while (avail < round_words) {
+			twist_parallel(st, p, tid);
+			for (int k = tid; k < p.nn; k += kSeqThreads) buf[avail + k] = mt_temper(st[k], p);
+			avail += p.nn;
+			__syncthreads();
+		}
+		const uint32_t* w = buf + tid * words_per_attempt;
+		bool accept;
+		double xd = 0, yd = 0, r2d = 0;
+		float xf = 0, yf = 0, r2f = 0;
+		if (SEM == ARMA_NOISE_DOUBLE) {
+			xd = 2.0 * canonical_from(w[0], w[1]) - 1.0;
+			yd = 2.0 * canonical_from(w[2], w[3]) - 1.0;
+			r2d = xd * xd + yd * yd;
+			accept = !(r2d > 1.0 || r2d == 0.0);
+		} else {
+			xf = 2.0f * canonical_from_f(w[0]) - 1.0f;
+			yf = 2.0f * canonical_from_f(w[1]) - 1.0f;
+			r2f = xf * xf + yf * yf;
+			accept = !(r2f > 1.0f || r2f == 0.0f);
+		}
+		const unsigned long long mask = __ballot(accept);
+		const int below = __popcll(mask & ((1ull << lane) - 1ull));
+		if (lane == 0) wave_tot[wave] = __popcll(mask);
+		__syncthreads();
+		int before = 0, total = 0;
+		for (int q = 0; q < kSeqThreads / 64; ++q) {
+			if (q < wave) before += wave_tot[q];
+			total += wave_tot[q];
+		}
+		if (accept) {
+			const long long pos = produced + 2ll * (before + below);
+			if (pos < n) {
+				if (SEM == ARMA_NOISE_DOUBLE) {
+					const double mult = sqrt(-2.0 * log(r2d) / r2d);
+					out[pos] = OutT(yd * mult * stddev + 0.0);
+					if (pos + 1 < n) out[pos + 1] = OutT(xd * mult * stddev + 0.0);
+				} else {
+					const float mult = sqrtf(-2.0f * logf(r2f) / r2f);
+					out[pos] = OutT(yf * mult * stddev_f + 0.0f);
+					if (pos + 1 < n) out[pos + 1] = OutT(xf * mult * stddev_f + 0.0f);
+				}
+			}
+		}
+		produced += 2ll * total;
+		__syncthreads();
+		// drop the consumed words
+		const int left = avail - round_words;
+		uint32_t keep[(kMaxNN + kSeqThreads - 1) / kSeqThreads];
+#pragma unroll
+		for (int j = 0; j < (kMaxNN + kSeqThreads - 1) / kSeqThreads; ++j) {
+			const int k = tid + j * kSeqThreads;
+			if (k < left) keep[j] = buf[round_words + k];
+		}
+		__syncthreads();
+#pragma unroll
+		for (int j = 0; j < (kMaxNN + kSeqThreads - 1) / kSeqThreads; ++j) {
+			const int k = tid + j * kSeqThreads;
+			if (k < left) buf[k] = keep[j];
+		}
+		avail = left;
+		__syncthreads();
+	}
+}
+
+int threads_for_state(int nn) {
+	// per-thread LDS state of nn words; at most 64 KiB per workgroup
+	int t = 256;
+	while (t > 1 && size_t(t) * nn * 4 > 65536) t >>= 1;
+	return t;
+}
+
+void validate_params(const arma_mt_params* mts, int n) {
+	for (int i = 0; i < n; ++i) {
+		const arma_mt_params& q = mts[i];
+		ARMA_HIP_REQUIRE(q.nn > 1 && q.nn <= kMaxNN && q.mm > 0 && q.mm < q.nn &&
+		                     q.shift0 >= 0 && q.shift0 < 32 && q.shift1 >= 0 && q.shift1 < 32 &&
+		                     q.shiftB >= 0 && q.shiftB < 32 && q.shiftC >= 0 && q.shiftC < 32,
+		                 ARMA_HIP_EINVAL, "bad Mersenne Twister parameter record");
+	}
+}
+
+}  // namespace
+
+void launch_white_noise(arma_hip_ctx* ctx, const arma_ar_desc* d, void* eps) {
+	ARMA_HIP_REQUIRE(d->varwn >= 0.0, ARMA_HIP_EINVAL, "variance is less than zero");
+	ARMA_HIP_REQUIRE(d->shape[0] > 0 && d->shape[1] > 0 && d->shape[2] > 0, ARMA_HIP_EINVAL,
+	                 "bad output grid shape");
+	const double stddev = std::sqrt(d->varwn);
+	const float stddev_f = std::sqrt(float(d->varwn));
+	const bool fp32 = d->precision == ARMA_FP32;
+	const int sem = d->noise_semantics;
+	ARMA_HIP_REQUIRE(sem == ARMA_NOISE_DOUBLE || sem == ARMA_NOISE_FLOAT, ARMA_HIP_EINVAL,
+	                 "bad noise semantics");
+	if (d->noise_mode == ARMA_NOISE_SEQUENTIAL) {
+		ARMA_HIP_REQUIRE(d->n_mts >= 1 && d->mts, ARMA_HIP_EPRNG, "bad number of MT configs");
+		validate_params(d->mts, 1);
+		const long long n = (long long)numel3(d->shape);
+		const uint32_t seed = d->seeds ? d->seeds[0] : 0u;
+		KernelTimer timer(ctx, "noise_sequential");
+		if (fp32) {
+			if (sem == ARMA_NOISE_DOUBLE)
+				noise_sequential_kernel<float, ARMA_NOISE_DOUBLE><<<1, kSeqThreads, 0, ctx->stream>>>(
+					d->mts[0], seed, stddev, stddev_f, (float*)eps, n);
+			else
+				noise_sequential_kernel<float, ARMA_NOISE_FLOAT><<<1, kSeqThreads, 0, ctx->stream>>>(
+					d->mts[0], seed, stddev, stddev_f, (float*)eps, n);
+		} else {
+			if (sem == ARMA_NOISE_DOUBLE)
+				noise_sequential_kernel<double, ARMA_NOISE_DOUBLE><<<1, kSeqThreads, 0, ctx->stream>>>(
+					d->mts[0], seed, stddev, stddev_f, (double*)eps, n);
+			else
+				noise_sequential_kernel<double, ARMA_NOISE_FLOAT><<<1, kSeqThreads, 0, ctx->stream>>>(
+					d->mts[0], seed, stddev, stddev_f, (double*)eps, n);
+		}
+		ARMA_HIP_CHECK(hipGetLastError());
+		return;
+	}
+	ARMA_HIP_REQUIRE(d->noise_mode == ARMA_NOISE_PARTITIONED, ARMA_HIP_EINVAL, "bad noise mode");
+	ARMA_HIP_REQUIRE(d->partition[0] > 0 && d->partition[1] > 0 && d->partition[2] > 0,
+	                 ARMA_HIP_EINVAL, "bad partition shape");
+	const int3 shape{d->shape[0], d->shape[1], d->shape[2]};
+	const int3 part{d->partition[0], d->partition[1], d->partition[2]};
+	const int3 np{div_ceil(shape.x, part.x), div_ceil(shape.y, part.y), div_ceil(shape.z, part.z)};
+	const int ntotal = np.x * np.y * np.z;
+	ARMA_HIP_REQUIRE(d->n_mts >= ntotal && d->mts, ARMA_HIP_EPRNG, "bad number of MT configs");
+	validate_params(d->mts, ntotal);
+	int nn_max = 0;
+	for (int i = 0; i < ntotal; ++i) nn_max = std::max(nn_max, int(d->mts[i].nn));
+	auto* dparams = (arma_mt_params*)ctx->params.get(sizeof(arma_mt_params) * ntotal);
+	upload(ctx, dparams, d->mts, sizeof(arma_mt_params) * ntotal);
+	uint32_t* dseeds = nullptr;
+	if (d->seeds) {
+		dseeds = (uint32_t*)ctx->seeds.get(sizeof(uint32_t) * ntotal);
+		upload(ctx, dseeds, d->seeds, sizeof(uint32_t) * ntotal);
+	}
+	const int tpb = std::min(threads_for_state(nn_max), 64);
+	const int blocks = div_ceil(ntotal, tpb);
+	const size_t lds = size_t(tpb) * nn_max * 4;
+	KernelTimer timer(ctx, "noise_partitioned");
+#define ARMA_LAUNCH_PART(OT, S)                                                                  \
+	noise_partitioned_kernel<OT, S><<<blocks, tpb, lds, ctx->stream>>>(dparams, dseeds, shape,   \
+	                                                                  part, np, stddev,         \
+	                                                                  stddev_f, (OT*)eps)
+	if (fp32) {
+		if (sem == ARMA_NOISE_DOUBLE) ARMA_LAUNCH_PART(float, ARMA_NOISE_DOUBLE);
+		else ARMA_LAUNCH_PART(float, ARMA_NOISE_FLOAT);
+	} else {
+		if (sem == ARMA_NOISE_DOUBLE) ARMA_LAUNCH_PART(double, ARMA_NOISE_DOUBLE);
+		else ARMA_LAUNCH_PART(double, ARMA_NOISE_FLOAT);
+	}
+#undef ARMA_LAUNCH_PART
+	ARMA_HIP_CHECK(hipGetLastError());
+}
+
+void launch_mt_raw(arma_hip_ctx* ctx, const arma_mt_params* mts, const uint32_t* seeds,
+                   int n_streams, int draws, uint32_t* out) {
+	ARMA_HIP_REQUIRE(n_streams > 0 && draws >= 0 && mts, ARMA_HIP_EINVAL, "bad stream count");
+	validate_params(mts, n_streams);
+	int nn_max = 0;
+	for (int i = 0; i < n_streams; ++i) nn_max = std::max(nn_max, int(mts[i].nn));
+	auto* dparams = (arma_mt_params*)ctx->params.get(sizeof(arma_mt_params) * n_streams);
+	upload(ctx, dparams, mts, sizeof(arma_mt_params) * n_streams);
+	uint32_t* dseeds = nullptr;
+	if (seeds) {
+		dseeds = (uint32_t*)ctx->seeds.get(sizeof(uint32_t) * n_streams);
+		upload(ctx, dseeds, seeds, sizeof(uint32_t) * n_streams);
+	}
+	const int tpb = std::min(threads_for_state(nn_max), 64);
+	mt_raw_kernel<<<div_ceil(n_streams, tpb), tpb, size_t(tpb) * nn_max * 4, ctx->stream>>>(
+		dparams, dseeds, n_streams, draws, out);
+	ARMA_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace arma_hip

@@ -1,0 +1,396 @@
+// FP64 model setup that the AR generator runs before the GPU work:
+// closed-form ACFs, Yule-Walker coefficient solvers, white-noise variance
+// and the partition shape.  These are host computations in the reference
+// too (src/generator/basic_arma_model.cc:51-101); they are small (at most a
+// few 271 x 271 matrices) and stay in FP64.
+//
+//   acf .................. src/acf.hh:18-62, src/bits/acf_wrapper.cc:41-70
+//   Choi order recursion . src/yule_walker.cc:420-539 (blocks :41-353)
+//   Gauss elimination .... src/generator/ar_model.cc:97-135,
+//                          src/generator/voodoo.cc:58-104
+//   chop ................. src/chop.cc:4-26
+//   white_noise_variance . src/generator/ar_model.cc:55-61
+//   partition shape ...... src/generator/ar_model.cc:23-50
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <tuple>
+#include <vector>
+
+#include "arma_hip.h"
+#include "common.hpp"
+
+namespace {
+
+// Dense row-major FP64 matrix.
+struct Matrix {
+	int rows = 0, cols = 0;
+	std::vector<double> v;
+	Matrix() = default;
+	Matrix(int r, int c) : rows(r), cols(c), v(size_t(r) * c, 0.0) {}
+	double& operator()(int i, int j) { return v[size_t(i) * cols + j]; }
+	double operator()(int i, int j) const { return v[size_t(i) * cols + j]; }
+};
+
+Matrix operator*(const Matrix& a, const Matrix& b) {
+	Matrix c(a.rows, b.cols);
+	for (int i = 0; i < a.rows; ++i)
+		for (int k = 0; k < a.cols; ++k) {
+			const double aik = a(i, k);
+			if (aik == 0.0) continue;
+			const double* brow = &b.v[size_t(k) * b.cols];
+			double* crow = &c.v[size_t(i) * c.cols];
+			for (int j = 0; j < b.cols; ++j) crow[j] += aik * brow[j];
+		}
+	return c;
+}
+
+Matrix operator-(Matrix a, const Matrix& b) {
+	for (size_t i = 0; i < a.v.size(); ++i) a.v[i] -= b.v[i];
+	return a;
+}
+
+double dot(const Matrix& a, const Matrix& b) {
+	double s = 0.0;
+	for (size_t i = 0; i < a.v.size(); ++i) s += a.v[i] * b.v[i];
+	return s;
+}
+
+// Inverse by LU with partial pivoting (getrf + getri).
+Matrix inverse(const Matrix& m) {
+	const int n = m.rows;
+	Matrix lu = m;
+	std::vector<int> piv(n);
+	for (int k = 0; k < n; ++k) {
+		int p = k;
+		for (int i = k + 1; i < n; ++i)
+			if (std::fabs(lu(i, k)) > std::fabs(lu(p, k))) p = i;
+		if (lu(p, k) == 0.0) throw arma_hip::Error(ARMA_HIP_ERUNTIME, "getrf error");
+		piv[k] = p;
+		if (p != k)
+			for (int j = 0; j < n; ++j) std::swap(lu(k, j), lu(p, j));
+		for (int i = k + 1; i < n; ++i) {
+			const double f = lu(i, k) / lu(k, k);
+			lu(i, k) = f;
+			for (int j = k + 1; j < n; ++j) lu(i, j) -= f * lu(k, j);
+		}
+	}
+	Matrix inv(n, n);
+	std::vector<double> col(n);
+	for (int c = 0; c < n; ++c) {
+		std::fill(col.begin(), col.end(), 0.0);
+		col[c] = 1.0;
+		for (int k = 0; k < n; ++k) std::swap(col[k], col[piv[k]]);
+		for (int i = 0; i < n; ++i)
+			for (int j = 0; j < i; ++j) col[i] -= lu(i, j) * col[j];
+		for (int i = n - 1; i >= 0; --i) {
+			for (int j = i + 1; j < n; ++j) col[i] -= lu(i, j) * col[j];
+			col[i] /= lu(i, i);
+		}
+		for (int i = 0; i < n; ++i) inv(i, c) = col[i];
+	}
+	return inv;
+}
+
+int shell(int a) { return 3 * a * a + 3 * a + 1; }
+
+// Offsets of the shell points of radius d around the corner (c, c, c) in the
+// order of the Choi block vectors: the corner, then for e = 1..d six runs of
+// e points each.
+struct Pt {
+	int i, j, k;
+};
+
+std::vector<Pt> shell_points(int d) {
+	std::vector<Pt> p;
+	p.reserve(shell(d));
+	p.push_back({0, 0, 0});
+	for (int e = 1; e <= d; ++e) {
+		for (int q = 0; q < e; ++q) p.push_back({e, q, 0});
+		for (int q = 0; q < e; ++q) p.push_back({e - q, e, 0});
+		for (int q = 0; q < e; ++q) p.push_back({0, e, q});
+		for (int q = 0; q < e; ++q) p.push_back({0, e - q, e});
+		for (int q = 0; q < e; ++q) p.push_back({q, 0, e});
+		for (int q = 0; q < e; ++q) p.push_back({e, 0, e - q});
+	}
+	return p;
+}
+
+// Row points of R_{a,b}: (a,a,a) followed by six runs per e, mirrored.
+std::vector<Pt> row_points(int a) {
+	std::vector<Pt> p;
+	p.reserve(shell(a));
+	p.push_back({a, a, a});
+	for (int e = 1; e <= a; ++e) {
+		for (int q = 0; q < e; ++q) p.push_back({a - e, a - q, a});
+		for (int q = 0; q < e; ++q) p.push_back({a - e + q, a - e, a});
+		for (int q = 0; q < e; ++q) p.push_back({a, a - e, a - q});
+		for (int q = 0; q < e; ++q) p.push_back({a, a - e + q, a - e});
+		for (int q = 0; q < e; ++q) p.push_back({a - q, a, a - e});
+		for (int q = 0; q < e; ++q) p.push_back({a - e, a, a - e + q});
+	}
+	return p;
+}
+
+class ChoiSolver {
+public:
+	ChoiSolver(const double* gamma, int n) : n_(n), rho_(size_t(n) * n * n) {
+		for (size_t q = 0; q < rho_.size(); ++q) rho_[q] = gamma[q] / gamma[0];
+		variance_ = gamma[0];
+	}
+
+	double rho(int i, int j, int k) const {
+		return rho_[(size_t(std::abs(i)) * n_ + std::abs(j)) * n_ + std::abs(k)];
+	}
+
+	// R_{a,b}: rows are rho vectors of the row points of a, of depth b.
+	Matrix R(int a, int b) const {
+		const auto rows = row_points(a);
+		const auto cols = shell_points(b);
+		Matrix m(int(rows.size()), int(cols.size()));
+		for (size_t r = 0; r < rows.size(); ++r)
+			for (size_t c = 0; c < cols.size(); ++c)
+				m(int(r), int(c)) = rho(rows[r].i - b + cols[c].i, rows[r].j - b + cols[c].j,
+				                        rows[r].k - b + cols[c].k);
+		return m;
+	}
+
+	Matrix R_b0(int a) const {
+		const auto rows = row_points(a);
+		Matrix m(int(rows.size()), 1);
+		for (size_t r = 0; r < rows.size(); ++r) m(int(r), 0) = rho(rows[r].i, rows[r].j, rows[r].k);
+		return m;
+	}
+
+	Matrix R_a0(int b) const {
+		const auto cols = shell_points(b);
+		Matrix m(int(cols.size()), 1);
+		for (size_t c = 0; c < cols.size(); ++c)
+			m(int(c), 0) = rho(-b + cols[c].i, -b + cols[c].j, -b + cols[c].k);
+		return m;
+	}
+
+	// Returns phi as an l x l x l array (before chopping) and the order l.
+	std::vector<double> solve(int max_order, double var_eps, bool determine, int& l_out,
+	                          double& varwn) {
+		const int M = std::min(max_order, n_ - 1);
+		if (M < 2) throw arma_hip::Error(ARMA_HIP_EINVAL, "AR order below 2 for Choi solver");
+		std::map<std::tuple<int, int, int>, Matrix> Phi;
+		std::vector<Matrix> Pi_l(M + 2), Pi_lp1(M + 2), Theta(M + 2);
+		const Matrix Rsup11 = inverse(R(1, 1));
+		Pi_l[1] = Rsup11 * R_b0(1);
+		double lambda = 1.0 - dot(R_a0(1), Pi_l[1]);
+		double var0 = variance_;
+		double var = variance_ * lambda;
+		Phi[{2, 1, 0}] = Rsup11 * R(1, 2);
+		bool changed = false;
+		int l = 1;
+		do {
+			++l;
+			var0 = var;
+			Matrix theta = R(l, l);
+			Matrix h = R_b0(l);
+			for (int m = 1; m <= l - 1; ++m) {
+				const Matrix Rlm = R(l, m);
+				theta = theta - Rlm * Phi[{l, m, 0}];
+				h = h - Rlm * Pi_l[m];
+			}
+			Theta[l] = inverse(theta);
+			Pi_lp1[l] = Theta[l] * h;
+			for (int a = 1; a <= l - 1; ++a) Pi_lp1[a] = Pi_l[a] - Phi[{l, a, 0}] * Pi_lp1[l];
+			lambda -= dot(h, Pi_lp1[l]);
+			var = variance_ * lambda;
+			if (l < M) {
+				Phi[{2, 1, l - 1}] = Rsup11 * R(1, l + 1);
+				for (int m = 2; m <= l; ++m) {
+					Matrix rhs = R(m, l + 1);
+					for (int q = 1; q <= m - 1; ++q) rhs = rhs - R(m, q) * Phi[{m, q, l - m + 1}];
+					Phi[{m + 1, m, l - m}] = Theta[m] * rhs;
+					for (int a = 1; a <= m - 1; ++a)
+						Phi[{m + 1, a, l - m}] = Phi[{m, a, l - m + 1}] - Phi[{m, a, 0}] * Phi[{m + 1, m, l - m}];
+				}
+			}
+			std::swap(Pi_l, Pi_lp1);
+			changed = !(determine && std::fabs(var - var0) < var_eps);
+		} while (l < M && changed);
+		const std::vector<Matrix>& beta = changed ? Pi_l : Pi_lp1;
+		varwn = changed ? var : var0;
+		std::vector<double> out(size_t(l) * l * l, 0.0);
+		for (int d = 1; d < l; ++d) {
+			const double* src = beta[d].v.data();
+			auto put = [&](int i, int j, int k) { out[(size_t(i) * l + j) * l + k] = *src++; };
+			put(d, d, d);
+			for (int e = 1; e <= d; ++e) {
+				for (int q = 0; q < e; ++q) put(d - e, d - q, d);
+				for (int q = 0; q < e; ++q) put(d - e + q, d - e, d);
+				for (int q = 0; q < e; ++q) put(d, d - e, d - q);
+				for (int q = 0; q < e; ++q) put(d, d - e + q, d - e);
+				for (int q = 0; q < e; ++q) put(d - q, d, d - e);
+				for (int q = 0; q < e; ++q) put(d - e, d, d - e + q);
+			}
+		}
+		l_out = l;
+		return out;
+	}
+
+private:
+	int n_;
+	std::vector<double> rho_;
+	double variance_ = 0.0;
+};
+
+void chop(const std::vector<double>& phi, const int shape[3], double eps, int out[3]) {
+	const int n0 = shape[0], n1 = shape[1], n2 = shape[2];
+	auto at = [&](int a, int b, int c) { return phi[(size_t(a) * n1 + b) * n2 + c]; };
+	auto small_plane = [&](int a0, int a1, int b0, int b1, int c0, int c1) {
+		for (int a = a0; a <= a1; ++a)
+			for (int b = b0; b <= b1; ++b)
+				for (int c = c0; c <= c1; ++c)
+					if (!(std::fabs(at(a, b, c)) < eps)) return false;
+		return true;
+	};
+	int k = n2 - 1;
+	while (k >= 1 && small_plane(0, n0 - 1, 0, n1 - 1, k, k)) --k;
+	int j = n1 - 1;
+	while (j >= 1 && small_plane(0, n0 - 1, j, j, 0, k)) --j;
+	int i = n0 - 1;
+	while (i >= 1 && small_plane(i, i, 0, j, 0, k)) --i;
+	out[0] = i + 1;
+	out[1] = j + 1;
+	out[2] = k + 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+int arma_hip_acf(int32_t func, const double delta[3], const int32_t shape[3], double amplitude,
+                 double velocity, const double alpha[3], const double beta[2], double* out) {
+	if (!delta || !shape || !alpha || !beta || !out) return ARMA_HIP_EINVAL;
+	if (func != ARMA_ACF_STANDING_WAVE && func != ARMA_ACF_PROPAGATING_WAVE) return ARMA_HIP_EINVAL;
+	for (int i = 0; i < shape[0]; ++i)
+		for (int j = 0; j < shape[1]; ++j)
+			for (int k = 0; k < shape[2]; ++k) {
+				const double decay =
+					std::exp(-(alpha[0] * i * delta[0] + alpha[1] * j * delta[1] + alpha[2] * k * delta[2]));
+				const double wave =
+					func == ARMA_ACF_STANDING_WAVE
+						? std::cos(velocity * i * delta[0]) * std::cos(beta[0] * j * delta[1]) *
+						      std::cos(beta[1] * k * delta[2])
+						: std::cos(velocity * i * delta[0] + beta[0] * j * delta[1] +
+						           beta[1] * k * delta[2]);
+				out[(size_t(i) * shape[1] + j) * shape[2] + k] = amplitude * decay * wave;
+			}
+	return ARMA_HIP_OK;
+}
+
+int arma_hip_yule_walker_choi(const double* acf, int32_t n, int32_t max_order, double var_eps,
+                              int32_t determine_order, int32_t do_chop, double chop_eps,
+                              double* phi_out, int32_t order_out[3], double* varwn_solver) {
+	if (!acf || !phi_out || !order_out || n < 3) return ARMA_HIP_EINVAL;
+	try {
+		ChoiSolver solver(acf, n);
+		int l = 0;
+		double var = 0.0;
+		std::vector<double> phi = solver.solve(max_order, var_eps, determine_order != 0, l, var);
+		int shp[3] = {l, l, l};
+		if (do_chop) chop(phi, shp, chop_eps, shp);
+		for (int a = 0; a < shp[0]; ++a)
+			for (int b = 0; b < shp[1]; ++b)
+				for (int c = 0; c < shp[2]; ++c)
+					phi_out[(size_t(a) * shp[1] + b) * shp[2] + c] = phi[(size_t(a) * l + b) * l + c];
+		order_out[0] = shp[0];
+		order_out[1] = shp[1];
+		order_out[2] = shp[2];
+		if (varwn_solver) *varwn_solver = var;
+		return ARMA_HIP_OK;
+	} catch (const arma_hip::Error& e) {
+		return e.code;
+	} catch (...) {
+		return ARMA_HIP_ERUNTIME;
+	}
+}
+
+int arma_hip_yule_walker_gauss(const double* acf, const int32_t acf_shape[3],
+                               const int32_t order[3], double* phi_out) {
+	if (!acf || !acf_shape || !order || !phi_out) return ARMA_HIP_EINVAL;
+	for (int d = 0; d < 3; ++d)
+		if (order[d] < 1 || order[d] > acf_shape[d]) return ARMA_HIP_EINVAL;
+	const int P = order[0] * order[1] * order[2];
+	const int m = P - 1;
+	phi_out[0] = 0.0;
+	if (m == 0) return ARMA_HIP_OK;
+	auto gam = [&](int a, int b, int c) {
+		return acf[(size_t(std::abs(a)) * acf_shape[1] + std::abs(b)) * acf_shape[2] + std::abs(c)];
+	};
+	auto coord = [&](int r, int& i, int& j, int& k) {
+		i = r / (order[1] * order[2]);
+		j = (r / order[2]) % order[1];
+		k = r % order[2];
+	};
+	Matrix A(m, m);
+	std::vector<double> b(m);
+	for (int r = 1; r < P; ++r) {
+		int ri, rj, rk;
+		coord(r, ri, rj, rk);
+		b[r - 1] = gam(ri, rj, rk);
+		for (int c = 1; c < P; ++c) {
+			int ci, cj, ck;
+			coord(c, ci, cj, ck);
+			A(r - 1, c - 1) = gam(ri - ci, rj - cj, rk - ck);
+		}
+	}
+	// Cholesky factorisation (the block-Toeplitz AC matrix is SPD for a valid ACF)
+	for (int j = 0; j < m; ++j) {
+		double d = A(j, j);
+		for (int k = 0; k < j; ++k) d -= A(j, k) * A(j, k);
+		if (!(d > 0.0)) return ARMA_HIP_ERUNTIME;
+		const double ljj = std::sqrt(d);
+		A(j, j) = ljj;
+		for (int i = j + 1; i < m; ++i) {
+			double s = A(i, j);
+			for (int k = 0; k < j; ++k) s -= A(i, k) * A(j, k);
+			A(i, j) = s / ljj;
+		}
+	}
+	for (int i = 0; i < m; ++i) {
+		for (int k = 0; k < i; ++k) b[i] -= A(i, k) * b[k];
+		b[i] /= A(i, i);
+	}
+	for (int i = m - 1; i >= 0; --i) {
+		for (int k = i + 1; k < m; ++k) b[i] -= A(k, i) * b[k];
+		b[i] /= A(i, i);
+	}
+	for (int i = 0; i < m; ++i) phi_out[i + 1] = b[i];
+	return ARMA_HIP_OK;
+}
+
+double arma_hip_white_noise_variance(const double* acf, const int32_t acf_shape[3],
+                                     const double* phi, const int32_t phi_shape[3]) {
+	double s = 0.0;
+	for (int a = 0; a < phi_shape[0]; ++a)
+		for (int b = 0; b < phi_shape[1]; ++b)
+			for (int c = 0; c < phi_shape[2]; ++c)
+				s += phi[(size_t(a) * phi_shape[1] + b) * phi_shape[2] + c] *
+				     acf[(size_t(a) * acf_shape[1] + b) * acf_shape[2] + c];
+	return acf[0] - s;
+}
+
+void arma_hip_partition_shape(const int32_t partition[3], const int32_t grid[3],
+                              const int32_t order[3], int32_t parallelism, int32_t out[3]) {
+	if (partition[0] * partition[1] * partition[2] > 0) {
+		for (int d = 0; d < 3; ++d) out[d] = partition[d];
+		return;
+	}
+	const int npar = std::max(1, 7 * int(std::cbrt(double(parallelism))));
+	for (int d = 0; d < 3; ++d) {
+		const int g1 = std::max(2 * order[d], 10);
+		const int g2 = grid[d] / npar + (grid[d] % npar ? 1 : 0);
+		out[d] = std::min(g1, g2) + std::abs(g1 - g2) / 2;
+	}
+}
+
+}  // extern "C"

@@ -1,0 +1,485 @@
+// Implementation of the HIP-framework drop-in driver (see arma_hip_driver.hh).
+#include "arma_hip_driver.hh"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <limits>
+#include <random>
+#include <sstream>
+
+namespace arma {
+
+// ---- errors: the reference's exception types (src/errors.hh:10-33) ----
+
+struct PRNG_error : std::runtime_error {
+	using std::runtime_error::runtime_error;
+};
+
+void check(int rc) {
+	if (rc == ARMA_HIP_OK) return;
+	const std::string msg = arma_hip_last_error();
+	switch (rc) {
+	case ARMA_HIP_EINVAL: throw std::invalid_argument(msg);
+	case ARMA_HIP_EPRNG: throw PRNG_error(msg);
+	default: throw std::runtime_error(msg);
+	}
+}
+
+Hip_context::Hip_context(int device) { check(arma_hip_init(device, &ctx_)); }
+Hip_context::~Hip_context() { arma_hip_destroy(ctx_); }
+
+// ---- input syntax (src/params.cc:9-57 and the operator>> of each type) ----
+
+namespace {
+
+std::istream& skip_comments(std::istream& in) {
+	char ch = 0;
+	while (in >> std::ws >> ch && ch == '#') {
+		in.ignore(std::numeric_limits<std::streamsize>::max(), '\n');
+		ch = 0;
+	}
+	if (ch != 0) in.putback(ch);
+	return in;
+}
+
+bool expect(std::istream& in, char c) {
+	char ch = 0;
+	if (in >> std::ws >> ch && ch == c) return true;
+	if (ch) in.putback(ch);
+	return false;
+}
+
+// Blitz TinyVector syntax "(a, b, c)".
+template <class V, size_t N>
+void read_vec(std::istream& in, std::array<V, N>& v, const std::string& name) {
+	if (!expect(in, '(')) throw std::runtime_error("bad vector for \"" + name + "\"");
+	for (size_t i = 0; i < N; ++i) {
+		if (!(in >> v[i])) throw std::runtime_error("bad vector for \"" + name + "\"");
+		if (i + 1 < N && !expect(in, ',')) throw std::runtime_error("bad vector for \"" + name + "\"");
+	}
+	if (!expect(in, ')')) throw std::runtime_error("bad vector for \"" + name + "\"");
+}
+
+// "(n0,n1,n2)" or "(n0,n1,n2) : (l0,l1,l2)"; the length defaults to n - 1.
+void read_grid(std::istream& in, Grid<3>& g, const std::string& name) {
+	read_vec(in, g.npoints, name);
+	if (expect(in, ':')) {
+		read_vec(in, g.length, name);
+	} else {
+		for (int d = 0; d < 3; ++d) g.length[d] = g.npoints[d] - 1;
+	}
+	for (int d = 0; d < 3; ++d)
+		if (g.npoints[d] <= 0 || !(g.length[d] > 0) || !std::isfinite(g.length[d]))
+			throw std::invalid_argument("bad grid \"" + name + "\"");
+}
+
+// "from (a,b) to (c,d) npoints (n,m)" in any order.
+void read_domain(std::istream& in, Domain<2>& d, const std::string& name) {
+	d.lbound = {0, 0};
+	d.ubound = {0, 0};
+	d.npoints = {1, 1};
+	for (int k = 0; k < 3; ++k) {
+		std::string word;
+		if (!(in >> std::ws >> word)) break;
+		if (word == "from") read_vec(in, d.lbound, name);
+		else if (word == "to") read_vec(in, d.ubound, name);
+		else if (word == "npoints") read_vec(in, d.npoints, name);
+		else throw std::runtime_error("bad prefix: " + word);
+	}
+	if (d.npoints[0] <= 0 || d.npoints[1] <= 0) throw std::invalid_argument("bad domain \"" + name + "\"");
+}
+
+void read_flags(std::istream& in, Output_flags& f) {
+	std::string line;
+	std::getline(in, line);
+	std::stringstream ss(line);
+	std::string name;
+	while (std::getline(ss, name, ',')) {
+		name.erase(0, name.find_first_not_of(" \t"));
+		name.erase(name.find_last_not_of(" \t\r") + 1);
+		if (name == "none") f.bits = 0;
+		else if (name == "surface") f.bits |= Output_flags::Surface;
+		else if (name == "acf") f.bits |= Output_flags::ACF;
+		else if (name == "csv") f.bits |= Output_flags::CSV;
+		else if (name == "blitz") f.bits |= Output_flags::Blitz;
+		else if (name == "binary") f.bits |= Output_flags::Binary;
+		else if (name == "summary" || name == "qq" || name == "waves") f.bits |= Output_flags::Summary;
+		else throw std::runtime_error("bad output flag: " + name);
+	}
+	// default format (src/output_flags.cc:39-51)
+	if (f.isset(Output_flags::Surface) && !f.isset(Output_flags::Blitz) &&
+	    !f.isset(Output_flags::CSV) && !f.isset(Output_flags::Binary))
+		f.bits |= Output_flags::Blitz;
+}
+
+template <class T>
+void read_value(std::istream& in, T& v, const std::string& name) {
+	if (!(in >> v)) throw std::runtime_error("bad value for \"" + name + "\"");
+}
+
+}  // namespace
+
+void parameter_map::read(std::istream& in) {
+	if (parens_ && !expect(in, '{')) throw std::runtime_error("Expecting \"{\".");
+	std::string name;
+	while (skip_comments(in)) {
+		if (parens_ && expect(in, '}')) break;
+		if (!(in >> std::ws && std::getline(in, name, '='))) break;
+		while (!name.empty() && name.back() <= ' ') name.pop_back();
+		auto it = r_.find(name);
+		if (it == r_.end()) throw std::runtime_error("Unknown parameter: \"" + name + "\".");
+		it->second(in, name);
+	}
+}
+
+// ---- AR model ----
+
+namespace generator {
+
+template <class T>
+void AR_model<T>::read(std::istream& in) {
+	std::string transform = "none";
+	bool have_acf = false;
+	parameter_map params(
+		{
+			{"algorithm",
+			 [&](std::istream& s, const std::string& n) {
+				 std::string a;
+				 read_value(s, a, n);
+				 if (a == "gauss_elimination") algorithm_ = AR_algorithm::Gauss_elimination;
+				 else if (a == "choi_recursive") algorithm_ = AR_algorithm::Choi;
+				 else throw std::runtime_error("bad algorithm");
+			 }},
+			{"partition", [&](std::istream& s, const std::string& n) { read_vec(s, partition_, n); }},
+			{"out_grid", [&](std::istream& s, const std::string& n) { read_grid(s, this->outgrid_, n); }},
+			{"no_seed", [&](std::istream& s, const std::string& n) { read_value(s, this->noseed_, n); }},
+			{"transform",
+			 [&](std::istream& s, const std::string& n) {
+				 read_value(s, transform, n);
+				 if (transform != "none")
+					 throw std::runtime_error("non-linear transform is not part of the HIP build");
+			 }},
+			{"acf",
+			 [&](std::istream& s, const std::string&) {
+				 // closed-form ACF block (src/bits/acf_wrapper.cc:41-70)
+				 std::string func;
+				 double amplitude = 1.0, velocity = 1.0;
+				 std::array<double, 3> alpha{0.06, 0.06, 0.06};
+				 std::array<double, 2> beta{0.8, 0.0};
+				 parameter_map p(
+					 {
+						 {"grid", [&](std::istream& q, const std::string& n) { read_grid(q, acf_grid_, n); }},
+						 {"func", [&](std::istream& q, const std::string& n) { read_value(q, func, n); }},
+						 {"amplitude", [&](std::istream& q, const std::string& n) { read_value(q, amplitude, n); }},
+						 {"velocity", [&](std::istream& q, const std::string& n) { read_value(q, velocity, n); }},
+						 {"alpha", [&](std::istream& q, const std::string& n) { read_vec(q, alpha, n); }},
+						 {"beta", [&](std::istream& q, const std::string& n) { read_vec(q, beta, n); }},
+					 },
+					 true);
+				 p.read(s);
+				 int f;
+				 if (func == "standing_wave") f = ARMA_ACF_STANDING_WAVE;
+				 else if (func == "propagating_wave") f = ARMA_ACF_PROPAGATING_WAVE;
+				 else throw std::runtime_error("bad ACF function name");
+				 if (acf_grid_.npoints[0] <= 0)
+					 throw std::runtime_error("the ACF block needs grid = (n0,n1,n2) : (l0,l1,l2)");
+				 const double delta[3] = {acf_grid_.delta(0), acf_grid_.delta(1), acf_grid_.delta(2)};
+				 acf_shape_ = acf_grid_.npoints;
+				 acf_.assign(size_t(acf_shape_[0]) * acf_shape_[1] * acf_shape_[2], 0.0);
+				 check(arma_hip_acf(f, delta, acf_shape_.data(), amplitude, velocity, alpha.data(),
+				                    beta.data(), acf_.data()));
+				 have_acf = true;
+			 }},
+			{"output", [&](std::istream& s, const std::string&) { read_flags(s, this->oflags_); }},
+			{"order", [&](std::istream& s, const std::string& n) { read_vec(s, order_, n); }},
+			{"validate", [&](std::istream& s, const std::string& n) { read_value(s, validate_, n); }},
+		},
+		true);
+	params.read(in);
+	if (!have_acf) throw std::runtime_error("the AR model needs an acf block");
+}
+
+template <class T>
+void AR_model<T>::determine_coefficients() {
+	// src/generator/ar_model.cc:81-154
+	if (algorithm_ == AR_algorithm::Choi) {
+		const int n = acf_shape_[0];
+		if (acf_shape_[1] != n || acf_shape_[2] != n) throw std::invalid_argument("ACF is not square");
+		int max_order = n - 1;
+		if (order_[0] > 0 && order_[1] > 0 && order_[2] > 0)
+			max_order = *std::max_element(order_.begin(), order_.end());
+		phi_.assign(size_t(std::max(max_order, 2)) * std::max(max_order, 2) * std::max(max_order, 2) +
+		                size_t(n) * n * n,
+		            0.0);
+		double var = 0.0;
+		check(arma_hip_yule_walker_choi(acf_.data(), n, max_order, 1e-6, 1, 1, 1e-10, phi_.data(),
+		                                phi_shape_.data(), &var));
+		phi_.resize(size_t(phi_shape_[0]) * phi_shape_[1] * phi_shape_[2]);
+		order_ = phi_shape_;
+		std::clog << "New AR model order = (" << order_[0] << ',' << order_[1] << ','
+		          << order_[2] << ")\n";
+	} else {
+		phi_shape_ = order_;
+		phi_.assign(size_t(order_[0]) * order_[1] * order_[2], 0.0);
+		check(arma_hip_yule_walker_gauss(acf_.data(), acf_shape_.data(), order_.data(), phi_.data()));
+	}
+	varwn_ = arma_hip_white_noise_variance(acf_.data(), acf_shape_.data(), phi_.data(),
+	                                       phi_shape_.data());
+}
+
+namespace {
+
+std::vector<arma_mt_params> read_mts(size_t n) {
+	// MT_CONFIG_FILE (src/config.hh.in:4)
+	std::vector<arma_mt_params> mts(n);
+	int32_t got = 0;
+	check(arma_hip_read_mt_config("mt.dat", int32_t(n), mts.data(), &got));
+	return mts;
+}
+
+std::vector<uint32_t> make_seeds(size_t n, bool noseed) {
+	// src/parallel_mt.cc:37-48
+	std::vector<uint32_t> seeds(n, 0u);
+	if (!noseed) {
+		std::random_device dev;
+		std::seed_seq seq{uint32_t(std::chrono::high_resolution_clock::now().time_since_epoch().count()),
+		                  uint32_t(dev())};
+		seq.generate(seeds.begin(), seeds.end());
+	}
+	return seeds;
+}
+
+// Device buffer owned by the host code.
+struct Device_buffer {
+	void* p = nullptr;
+	explicit Device_buffer(size_t bytes) {
+		if (hipMalloc(&p, bytes) != hipSuccess) throw std::runtime_error("hipMalloc failed");
+	}
+	~Device_buffer() {
+		if (p) (void)hipFree(p);
+	}
+};
+
+}  // namespace
+
+template <class T>
+Array3D<T> AR_model<T>::do_generate() {
+	// the HIP body of AR_model<T>::do_generate (src/generator/ar_model.cc:189-197)
+	if (varwn_ < 0) throw std::invalid_argument("variance is less than zero");
+	const Shape3D shape = this->outgrid_.npoints;
+	const bool partitioned = partition_[0] > 0 && partition_[1] > 0 && partition_[2] > 0;
+	size_t nstreams = 1;
+	if (partitioned)
+		for (int d = 0; d < 3; ++d) nstreams *= size_t((shape[d] + partition_[d] - 1) / partition_[d]);
+	const std::vector<arma_mt_params> mts = read_mts(nstreams);
+	const std::vector<uint32_t> seeds = make_seeds(nstreams, this->noseed_);
+	arma_ar_desc d;
+	std::memset(&d, 0, sizeof d);
+	for (int k = 0; k < 3; ++k) {
+		d.shape[k] = shape[k];
+		d.phi_shape[k] = phi_shape_[k];
+		d.partition[k] = partition_[k];
+	}
+	d.phi = phi_.data();
+	d.varwn = varwn_;
+	d.noise_mode = partitioned ? ARMA_NOISE_PARTITIONED : ARMA_NOISE_SEQUENTIAL;
+	d.noise_semantics = ARMA_NOISE_DOUBLE;
+	d.mts = mts.data();
+	d.n_mts = int32_t(mts.size());
+	d.seeds = seeds.data();
+	d.precision = sizeof(T) == 4 ? ARMA_FP32 : ARMA_FP64;
+	const char* kern = std::getenv("ARMA_AR_KERNEL");
+	d.algorithm = (kern && std::string(kern) == "plain") ? ARMA_AR_PLAIN : ARMA_AR_WAVEFRONT;
+	Array3D<T> zeta(shape);
+	Device_buffer buf(zeta.size() * sizeof(T));
+	check(arma_hip_ar_generate(ctx_->get(), &d, buf.p, 1));
+	// post-rescale of Basic_ARMA_model::generate (src/generator/basic_arma_model.cc:102-106)
+	double factor = 0.0;
+	check(arma_hip_rescale(ctx_->get(), buf.p, shape.data(), d.precision, acf_[0], &factor));
+	check(arma_hip_synchronize(ctx_->get()));
+	if (hipMemcpy(zeta.data.data(), buf.p, zeta.size() * sizeof(T), hipMemcpyDeviceToHost) != hipSuccess)
+		throw std::runtime_error("hipMemcpy failed");
+	return zeta;
+}
+
+template <class T>
+Array3D<T> AR_model<T>::generate() {
+	// src/generator/basic_arma_model.cc:51-113 with the closed-form ACF
+	if (!(order_[0] > 0 && order_[1] > 0 && order_[2] > 0)) order_ = acf_shape_;
+	for (int d = 0; d < 3; ++d)
+		this->outgrid_.length[d] = acf_grid_.delta(d) * this->outgrid_.num_patches(d) * 2.0 * M_PI;
+	determine_coefficients();
+	std::clog << "White noise variance = " << varwn_ << '\n';
+	if (validate_)
+		std::clog << "validate: the process root check needs GSL and is not part of the HIP build\n";
+	return do_generate();
+}
+
+}  // namespace generator
+
+// ---- velocity potential ----
+
+namespace velocity {
+
+template <class T>
+void Velocity_potential_solver<T>::read(std::istream& in) {
+	parameter_map params(
+		{
+			{"wnmax", [&](std::istream& s, const std::string& n) { read_domain(s, wnmax_, n); }},
+			{"depth", [&](std::istream& s, const std::string& n) { read_value(s, depth_, n); }},
+			{"domain", [&](std::istream& s, const std::string& n) { read_domain(s, domain_, n); }},
+		},
+		true);
+	params.read(in);
+	if (domain_.lbound[1] < -depth_) throw std::invalid_argument("min(z) is less than -h");
+}
+
+template <class T>
+Array4D<T> Linear_solver<T>::operator()(const Array3D<T>& zeta, const Grid<3>& grid) {
+	arma_pot_desc d;
+	std::memset(&d, 0, sizeof d);
+	for (int k = 0; k < 3; ++k) {
+		d.shape[k] = zeta.shape[k];
+		d.delta[k] = grid.delta(k);
+	}
+	d.depth = this->depth_;
+	for (int k = 0; k < 2; ++k) {
+		d.lbound[k] = this->domain_.lbound[k];
+		d.ubound[k] = this->domain_.ubound[k];
+		d.npoints[k] = this->domain_.npoints[k];
+	}
+	d.precision = sizeof(T) == 4 ? ARMA_FP32 : ARMA_FP64;
+	Array4D<T> out({this->domain_.npoints[0], this->domain_.npoints[1], zeta.shape[1], zeta.shape[2]});
+	check(arma_hip_linear_potential(ctx_->get(), &d, zeta.data.data(), out.data.data(), 0, nullptr));
+	return out;
+}
+
+}  // namespace velocity
+
+// ---- driver ----
+
+template <class T>
+void ARMA_driver<T>::open(const std::string& filename) {
+	std::ifstream cfg(filename);
+	if (!cfg.is_open()) throw std::runtime_error("Cannot open input file \"" + filename + "\".");
+	std::clog << "Input file = " << filename << '\n';
+	read(cfg);
+}
+
+template <class T>
+void ARMA_driver<T>::read(std::istream& in) {
+	// src/arma_driver.cc:133-178 with Object_wrapper (src/bits/object_wrapper.hh:31-46)
+	parameter_map params(
+		{
+			{"model",
+			 [&](std::istream& s, const std::string&) {
+				 std::string name;
+				 s >> std::ws >> name;
+				 auto it = models_.find(name);
+				 if (it == models_.end()) throw std::runtime_error("Invalid object: " + name);
+				 model_ = it->second();
+				 model_->read(s);
+			 }},
+			{"velocity_potential_solver",
+			 [&](std::istream& s, const std::string&) {
+				 std::string name;
+				 s >> std::ws >> name;
+				 auto it = solvers_.find(name);
+				 if (it == solvers_.end()) throw std::runtime_error("Invalid object: " + name);
+				 solver_ = it->second();
+				 solver_->read(s);
+			 }},
+		},
+		false);
+	params.read(in);
+	if (!solver_) throw std::runtime_error("bad solver");
+	if (!model_) throw std::runtime_error("bad generator");
+}
+
+template <class T>
+void ARMA_driver<T>::generate_wavy_surface() {
+	zeta_ = model_->generate();
+}
+
+template <class T>
+void ARMA_driver<T>::compute_velocity_potentials() {
+	vpotentials_ = (*solver_)(zeta_, model_->grid());
+}
+
+namespace {
+
+// network byte order (src/io/binary_stream.hh:43-70)
+template <class T>
+void write_big_endian(std::ostream& out, const std::vector<T>& v) {
+	std::vector<unsigned char> buf(v.size() * sizeof(T));
+	for (size_t i = 0; i < v.size(); ++i) {
+		unsigned char b[sizeof(T)];
+		std::memcpy(b, &v[i], sizeof(T));
+		for (size_t k = 0; k < sizeof(T); ++k) buf[i * sizeof(T) + k] = b[sizeof(T) - 1 - k];
+	}
+	out.write((const char*)buf.data(), std::streamsize(buf.size()));
+}
+
+}  // namespace
+
+template <class T>
+void ARMA_driver<T>::write_all() {
+	// src/arma_driver.cc:29-69,121-130
+	const Output_flags f = model_->oflags();
+	if (!f.isset(Output_flags::Surface)) return;
+	const Grid<3>& g = model_->grid();
+	const int nt = zeta_.shape[0], nx = zeta_.shape[1], ny = zeta_.shape[2];
+	if (f.isset(Output_flags::Binary)) {
+		std::ofstream out("zeta.bin", std::ios::binary);
+		write_big_endian(out, zeta_.data);
+	}
+	if (f.isset(Output_flags::CSV)) {
+		std::ofstream out("zeta.csv");
+		out.precision(std::numeric_limits<T>::max_digits10);
+		out << "t,x,y,z\n";
+		for (int i = 0; i < nt; ++i)
+			for (int j = 0; j < nx; ++j)
+				for (int k = 0; k < ny; ++k)
+					out << i << ',' << g.delta(1) * j << ',' << g.delta(2) * k << ','
+					    << zeta_.data[(size_t(i) * nx + j) * ny + k] << '\n';
+		const auto& dom = solver_->domain();
+		const int ntp = vpotentials_.shape[0], nz = vpotentials_.shape[1];
+		std::ofstream po("phi.csv");
+		po.precision(std::numeric_limits<T>::max_digits10);
+		po << "t,z,x,y,phi\n";
+		for (int i = 0; i < ntp; ++i)
+			for (int j = 0; j < nz; ++j)
+				for (int k = 0; k < nx; ++k)
+					for (int l = 0; l < ny; ++l)
+						po << dom(i, 0) << ',' << dom(j, 1) << ',' << g.delta(1) * k << ','
+						   << g.delta(2) * l << ','
+						   << vpotentials_.data[((size_t(i) * nz + j) * nx + k) * ny + l] << '\n';
+	}
+	if (f.isset(Output_flags::Blitz)) {
+		// Blitz++ text layout: the extents, then the values in brackets
+		std::ofstream out("zeta");
+		out.precision(std::numeric_limits<T>::max_digits10);
+		out << "(0," << nt - 1 << ") x (0," << nx - 1 << ") x (0," << ny - 1 << ")\n[ ";
+		for (size_t q = 0; q < zeta_.data.size(); ++q) out << zeta_.data[q] << ' ';
+		out << "]\n";
+	}
+}
+
+template class generator::AR_model<double>;
+template class generator::AR_model<float>;
+template class velocity::Velocity_potential_solver<double>;
+template class velocity::Velocity_potential_solver<float>;
+template class velocity::Linear_solver<double>;
+template class velocity::Linear_solver<float>;
+template class ARMA_driver<double>;
+template class ARMA_driver<float>;
+
+}  // namespace arma

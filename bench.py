@@ -1,0 +1,269 @@
+#!/usr/bin/env python3
+"""Benchmark of the ARMA hot path on one MI355X.
+
+One step = one pass of the hot path over one synthetic surface:
+white noise (dynamic-creator MT streams, one per partition) -> 3-D AR
+recurrence (persistent wavefront kernel) -> post-rescale -> linear velocity
+potential (batched FFTs).  The workload is the AR + linear potential
+configuration: propagating-wave ACF exp(-0.06(|t|+|x|+|y|)) cos(0.8t - 0.8x)
+on a 10^3 unit grid, Choi coefficients of order 7, surface 1024x256x256,
+potential on 256x256 at 16 z levels in [-10, 0], float64.
+
+Prints ONE JSON line (rank 0).  value = (surface points + potential points)
+per second of the whole step; the two stage rates are reported beside it.
+The roofline object is computed for the kernel with the largest share of the
+timed region, from HIP events recorded on the stream the kernels run on.
+The CPU baseline is the FP64 oracle (a single-threaded C restatement of the
+reference's CPU path) timed on a bounded sample of the same workload.
+"""
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
+
+WORKLOADS = {
+    # AR + linear potential, float64 (the configuration the metric is quoted on)
+    "ar-potential-1024x256x256-fp64": dict(
+        acf=("propagating", 0.8, (-0.8, 0.0)), shape=(1024, 256, 256), order=7,
+        partition=(16, 32, 32), seed=0, depth=12.0, lbound=(0.0, -10.0),
+        ubound=(1023.0, 0.0), npoints=(1024, 16), precision="fp64"),
+    # precision variant: FP32 storage, compensated accumulation, FP64 setup
+    "ar-potential-2048x512x512-fp32": dict(
+        acf=("propagating", 0.8, (-0.8, 0.0)), shape=(2048, 512, 512), order=7,
+        partition=(16, 32, 32), seed=1, depth=12.0, lbound=(0.0, -10.0),
+        ubound=(2047.0, 0.0), npoints=(2048, 32), precision="fp32"),
+    # CPU-runnable plain AR case (no potential)
+    "ar-standing-128x64x64-fp64": dict(
+        acf=("standing", 0.8, (0.8, 0.8)), shape=(128, 64, 64), order=7,
+        partition=(16, 16, 16), seed=0, depth=None, precision="fp64"),
+}
+DEFAULT = "ar-potential-1024x256x256-fp64"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def setup_model(arma, w):
+    """FP64 host setup, as the reference generator does before do_generate."""
+    kind, velocity, beta = w["acf"]
+    func = arma._abi.ACF_STANDING_WAVE if kind == "standing" else arma._abi.ACF_PROPAGATING_WAVE
+    acf = arma._abi.acf(func, (10, 10, 10), velocity=velocity, alpha=(0.06, 0.06, 0.06), beta=beta)
+    phi, _ = arma._abi.yule_walker_choi(acf, w["order"])
+    varwn = arma._abi.white_noise_variance(acf, phi)
+    return acf, phi, varwn
+
+
+def cpu_baseline(w, phi, varwn, params):
+    """Oracle timings on a bounded sample: the first slices of the surface and
+    a few time slices of the potential.  Single-threaded."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import pyoracle as O
+    nt, nx, ny = w["shape"]
+    part = w["partition"]
+    ts = min(nt, 2 * part[0] if nx * ny >= 65536 else nt)
+    sample = (ts, nx, ny)
+    np_ = [math.ceil(sample[d] / part[d]) for d in range(3)]
+    nstreams = np_[0] * np_[1] * np_[2]
+    plist = [O.MtParams(*params[i].as_tuple()) for i in range(nstreams)]
+    t0 = time.perf_counter()
+    eps = O.noise_partitioned(plist, None, sample, part, math.sqrt(varwn))
+    zeta = O.ar_generate(eps, phi)
+    t_surf = time.perf_counter() - t0
+    surf_pts = ts * nx * ny
+    res = {"surface_pts_per_s": surf_pts / t_surf}
+    desc = f"noise+AR on {ts}x{nx}x{ny}"
+    if w.get("depth") is not None:
+        nz = w["npoints"][1]
+        tp = 4
+        z = O.ar_generate(O.noise_partitioned(plist, None, sample, part, math.sqrt(varwn)), phi)
+        z = np.nan_to_num(z[:max(tp, 3)], nan=0.0, posinf=0.0, neginf=0.0)
+        t0 = time.perf_counter()
+        O.linear_potential(z, (2 * math.pi,) * 3, w["depth"], (0.0, w["lbound"][1]),
+                           (float(tp - 1), w["ubound"][1]), (tp, nz))
+        t_pot = time.perf_counter() - t0
+        res["potential_pts_per_s"] = tp * nz * nx * ny / t_pot
+        desc += f"; potential of {tp} time slices x {nz} z levels"
+        value = (surf_pts + tp * nz * nx * ny) / (t_surf + t_pot)
+    else:
+        value = res["surface_pts_per_s"]
+    del zeta
+    return {"value": value, "unit": "pts/s", "cores": 1, "kind": "port", "sample": desc, **res}
+
+
+def parse_profile(ctx):
+    from arma_amd import _abi
+    buf = C.create_string_buffer(1 << 16)
+    _abi.check(_abi.lib().arma_hip_profile_report(ctx.h, buf, len(buf), 1))
+    out = {}
+    for line in buf.value.decode().splitlines():
+        parts = line.split()
+        if len(parts) == 4 and parts[0] == "prfl":
+            out[parts[1]] = (float(parts[2].rstrip("us")), int(parts[3]))
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
+    ap.add_argument("--ar", default="wavefront", choices=["wavefront", "plain"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the AR path shards by "
+              "x-bands and the potential by time slices, neither is built for several GPUs here.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import arma_amd
+    from arma_amd import _abi
+
+    w = WORKLOADS[args.workload]
+    fp32 = w["precision"] == "fp32"
+    prec = arma_amd.FP32 if fp32 else arma_amd.FP64
+    tdt = torch.float32 if fp32 else torch.float64
+    esize = 4 if fp32 else 8
+    nt, nx, ny = w["shape"]
+    log(f"bench: workload {args.workload}")
+    acf, phi, varwn = setup_model(arma_amd, w)
+    log(f"bench: phi shape {phi.shape}, white-noise variance {varwn:.6g}")
+    part = w["partition"]
+    nstreams = math.prod(math.ceil(w["shape"][d] / part[d]) for d in range(3))
+    t0 = time.time()
+    params = arma_amd.dcmt_create_many(521, nstreams, 20240611, min(16, os.cpu_count() or 1))
+    log(f"bench: created {nstreams} MT parameter sets in {time.time() - t0:.1f}s")
+    seeds = np.full(nstreams, w["seed"], np.uint32)
+
+    ctx = arma_amd.Context(0)
+    algo = arma_amd.AR_WAVEFRONT if args.ar == "wavefront" else arma_amd.AR_PLAIN
+    desc = ctx.ar_desc(w["shape"], phi, varwn, params, seeds=seeds,
+                       noise_mode=arma_amd.NOISE_PARTITIONED, partition=part, precision=prec,
+                       algorithm=algo)
+    zeta = torch.empty(w["shape"], dtype=tdt, device="cuda:0")
+    has_pot = w.get("depth") is not None
+    if has_pot:
+        nz = w["npoints"][1]
+        pdesc = ctx.pot_desc(w["shape"], (2 * math.pi,) * 3, w["depth"], w["lbound"],
+                             w["ubound"], w["npoints"], prec)
+        phi_out = torch.empty((w["npoints"][0], nz, nx, ny), dtype=tdt, device="cuda:0")
+    gamma0 = float(acf[0, 0, 0])
+
+    ev = {k: (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for k in ("surface", "potential")}
+    stage_ms = {"surface": 0.0, "potential": 0.0}
+
+    def step(timed):
+        if timed:
+            ev["surface"][0].record()
+        ctx.ar_generate(desc, zeta)
+        ctx.rescale(zeta, w["shape"], prec, gamma0)
+        if timed:
+            ev["surface"][1].record()
+        if has_pot:
+            if timed:
+                ev["potential"][0].record()
+            ctx.linear_potential(pdesc, zeta, phi_out)
+            if timed:
+                ev["potential"][1].record()
+
+    for i in range(args.warmup):
+        step(False)
+        torch.cuda.synchronize()
+        log(f"bench: warmup step {i + 1}/{args.warmup} done")
+    _abi.check(_abi.lib().arma_hip_profile(ctx.h, 1))
+    parse_profile(ctx)  # drop anything recorded before the timed region
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        step(True)
+        torch.cuda.synchronize()
+        stage_ms["surface"] += ev["surface"][0].elapsed_time(ev["surface"][1])
+        if has_pot:
+            stage_ms["potential"] += ev["potential"][0].elapsed_time(ev["potential"][1])
+        log(f"bench: step {i + 1}/{args.steps} done")
+    torch.cuda.synchronize()
+    t_total = time.perf_counter() - t_start
+    prof = parse_profile(ctx)
+    _abi.check(_abi.lib().arma_hip_profile(ctx.h, 0))
+
+    ms_per_step = 1000.0 * t_total / args.steps
+    surf_pts = nt * nx * ny
+    pot_pts = (w["npoints"][0] * nz * nx * ny) if has_pot else 0
+    value = (surf_pts + pot_pts) / (t_total / args.steps)
+    surf_rate = surf_pts / (stage_ms["surface"] / 1000.0 / args.steps)
+    pot_rate = pot_pts / (stage_ms["potential"] / 1000.0 / args.steps) if has_pot else None
+
+    # per-kernel roofline figures (algorithmic work per launch / mean launch time)
+    taps = int(np.prod(phi.shape))
+    kernels = {}
+    for name, (us, cnt) in prof.items():
+        mean_s = us / cnt * 1e-6
+        entry = {"mean_us": us / cnt, "launches": cnt, "share": us / 1e3 / (t_total * 1e3)}
+        if name == "ar_wavefront" or name == "ar_plain":
+            flops = 2.0 * taps * surf_pts
+            peak = FP32_PEAK_TFLOPS if fp32 else FP64_PEAK_TFLOPS
+            entry.update(bound="mfma", unit="TFLOP/s", achieved=flops / mean_s / 1e12, peak=peak)
+        elif name == "noise_partitioned" or name == "noise_sequential":
+            entry.update(bound="hbm", unit="GB/s", achieved=surf_pts * esize / mean_s / 1e9,
+                         peak=HBM_PEAK_GBS)
+        elif name == "pot_inv_rows":
+            chunk_pts = pot_pts / cnt * args.steps
+            entry.update(bound="hbm", unit="GB/s", achieved=chunk_pts * esize / mean_s / 1e9,
+                         peak=HBM_PEAK_GBS)
+        elif name == "rescale_scale":
+            entry.update(bound="hbm", unit="GB/s", achieved=2 * surf_pts * esize / mean_s / 1e9,
+                         peak=HBM_PEAK_GBS)
+        kernels[name] = entry
+    dom = max(kernels, key=lambda k: kernels[k]["share"]) if kernels else None
+    roofline = None
+    if dom and "achieved" in kernels[dom]:
+        k = kernels[dom]
+        roofline = {"bound": k["bound"], "achieved": round(k["achieved"], 3), "peak": k["peak"],
+                    "unit": k["unit"], "frac": round(k["achieved"] / k["peak"], 4),
+                    "traffic": None, "kernel": dom}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("bench: timing the CPU oracle on a sample")
+        try:
+            cpu = cpu_baseline(w, phi, varwn, params)
+        except Exception as e:  # report, never fail the bench line for it
+            cpu = {"error": str(e)}
+
+    line = {
+        "metric": "surface pts/s + potential pts/s (FP32 vs FP64) at RMS<=1e-5; p50/p99 latency",
+        "value": value, "unit": "pts/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32" if fp32 else "f64",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "surface": list(w["shape"]),
+                   "phi_shape": list(phi.shape), "partition": list(part),
+                   "noise_streams": nstreams, "ar_kernel": args.ar,
+                   "potential": list(w["npoints"]) + [nx, ny] if has_pot else None,
+                   "parallelism": "single-gpu"},
+        "surface_pts_per_s": surf_rate, "potential_pts_per_s": pot_rate,
+        "stage_ms_per_step": {k: v / args.steps for k, v in stage_ms.items()},
+        "roofline": roofline, "kernels": kernels, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

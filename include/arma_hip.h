@@ -1,0 +1,213 @@
+/*
+ * arma_hip.h -- C ABI of the MI355X-native ARMA hot path (libarma_hip.so).
+ *
+ * The entry points replace, one for one, the framework-specific bodies of
+ * the reference generator and velocity solver:
+ *
+ *   arma_hip_ar_generate        AR_model<T>::do_generate()
+ *                               (src/generator/ar_model_sequential.cc:3-11,
+ *                                src/generator/ar_model_opencl.cc:46-121,
+ *                                src/generator/ar_model_parallel.cc:76-201;
+ *                                selected at src/generator/ar_model.cc:189-197)
+ *   arma_hip_white_noise        Basic_ARMA_model<T>::generate_white_noise()
+ *                               (src/generator/basic_arma_model.cc:36-48) and
+ *                               the per-partition ::generate_white_noise
+ *                               (src/generator/ar_model_wn.cc:3-18)
+ *   arma_hip_ar_recurrence      ar_generate_surface over a whole grid
+ *                               (src/generator/ar_model_surf.cc:3-36,
+ *                                src/kernels/ar_generate_surface.cl:6-49)
+ *   arma_hip_rescale            the post-rescale of Basic_ARMA_model::generate
+ *                               (src/generator/basic_arma_model.cc:102-106)
+ *   arma_hip_linear_potential   Velocity_potential_solver<T>::operator() with
+ *                               Linear_solver<T>
+ *                               (src/velocity/basic_solver.cc:39-77,
+ *                                src/velocity/linear_solver.cc:20-184)
+ *   arma_hip_read_mt_config     read_parallel_mts / mt_config >>
+ *                               (src/parallel_mt.cc:28-59,
+ *                                src/parallel_mt.hh:90-97)
+ *   arma_hip_write_mt_config    mt_config << (src/parallel_mt.hh:99-104)
+ *   arma_hip_dcmt_create        parallel_mt_seq<p,w>::operator()
+ *                               (src/parallel_mt.hh:177-210, src/dcmt.cc:15-40)
+ *   arma_hip_init / _destroy    arma_init / arma_finalise
+ *                               (src/common_main.hh:48-75)
+ *
+ * Conventions: every function returns ARMA_HIP_OK (0) or a negative error
+ * code; arma_hip_last_error() returns a thread-local message for the last
+ * failure.  Arrays are row-major (t, x, y) exactly as Blitz lays out
+ * Array3D<T> (src/types.hh:25-43); potentials are (t, z, x, y).  A context
+ * owns one HIP stream and the device buffers it allocates; it is used by one
+ * host thread at a time.
+ */
+#ifndef ARMA_HIP_H
+#define ARMA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ARMA_HIP_OK 0
+#define ARMA_HIP_EINVAL (-1)      /* std::invalid_argument in the reference */
+#define ARMA_HIP_EPRNG (-2)       /* arma::PRNG_error (src/errors.hh:10-33) */
+#define ARMA_HIP_ERUNTIME (-3)    /* std::runtime_error */
+#define ARMA_HIP_EDEVICE (-4)     /* HIP runtime failure */
+#define ARMA_HIP_ENOMEM (-5)
+
+/* Fields of the dynamic-creator mt_struct, in file order
+   (src/parallel_mt.hh:109-123); the state words are not part of it because
+   every stream is re-seeded with sgenrand_mt (src/parallel_mt.hh:228-260). */
+typedef struct arma_mt_params {
+	uint32_t aaa;
+	int32_t mm, nn, rr, ww;
+	uint32_t wmask, umask, lmask;
+	int32_t shift0, shift1, shiftB, shiftC;
+	uint32_t maskB, maskC;
+} arma_mt_params;
+
+enum arma_precision {
+	ARMA_FP64 = 0,        /* real_type=double */
+	ARMA_FP32 = 1         /* FP32 storage, compensated FP32 accumulation */
+};
+
+enum arma_noise_mode {
+	ARMA_NOISE_SEQUENTIAL = 0,   /* stream 0 over the whole grid (framework=none) */
+	ARMA_NOISE_PARTITIONED = 1   /* stream p fills partition p (row-major) */
+};
+
+enum arma_noise_semantics {
+	ARMA_NOISE_DOUBLE = 0,  /* normal_distribution<double>; FP32 output = float(FP64 noise) */
+	ARMA_NOISE_FLOAT = 1    /* normal_distribution<float> (real_type=float build) */
+};
+
+enum arma_ar_algorithm {
+	ARMA_AR_PLAIN = 0,      /* one work-item per partition, DAG levels as launches */
+	ARMA_AR_WAVEFRONT = 1   /* persistent wavefront over (x,y) tiles marching in t */
+};
+
+typedef struct arma_ar_desc {
+	int32_t shape[3];            /* nt, nx, ny (out_grid num_points) */
+	const double* phi;           /* host FP64 coefficients, row-major */
+	int32_t phi_shape[3];
+	double varwn;                /* white-noise variance */
+	int32_t noise_mode;          /* enum arma_noise_mode */
+	int32_t noise_semantics;     /* enum arma_noise_semantics */
+	int32_t partition[3];        /* partition shape (AR "partition" key) */
+	const arma_mt_params* mts;   /* host array of n_mts parameter sets */
+	int32_t n_mts;
+	const uint32_t* seeds;       /* n_mts seeds; NULL = all zero (no_seed) */
+	int32_t precision;           /* enum arma_precision */
+	int32_t algorithm;           /* enum arma_ar_algorithm */
+	int32_t tile[3];             /* tile hint for ARMA_AR_WAVEFRONT (0 = auto) */
+} arma_ar_desc;
+
+typedef struct arma_pot_desc {
+	int32_t shape[3];            /* zeta extent nt, nx, ny */
+	double delta[3];             /* zeta grid spacing (grid().delta()) */
+	double depth;                /* water depth h */
+	double lbound[2];            /* domain from (t0, z0) */
+	double ubound[2];            /* domain to (t1, z1) */
+	int32_t npoints[2];          /* domain npoints (nt_out, nz) */
+	int32_t precision;           /* enum arma_precision */
+} arma_pot_desc;
+
+typedef struct arma_hip_ctx arma_hip_ctx;
+
+int arma_hip_init(int device, arma_hip_ctx** ctx);
+void arma_hip_destroy(arma_hip_ctx* ctx);
+const char* arma_hip_last_error(void);
+/* Use a caller-owned hipStream_t instead of the context's own stream. */
+int arma_hip_set_stream(arma_hip_ctx* ctx, void* stream);
+void* arma_hip_get_stream(arma_hip_ctx* ctx);
+int arma_hip_synchronize(arma_hip_ctx* ctx);
+
+/* White noise eps ~ N(0, varwn) into eps (device pointer, FP64 or FP32 per
+   desc->precision).  Returns ARMA_HIP_EPRNG when n_mts is smaller than the
+   number of streams the mode needs, ARMA_HIP_EINVAL when varwn < 0. */
+int arma_hip_white_noise(arma_hip_ctx* ctx, const arma_ar_desc* desc, void* eps);
+
+/* The AR recurrence in place over zeta (device pointer holding the noise). */
+int arma_hip_ar_recurrence(arma_hip_ctx* ctx, const arma_ar_desc* desc, void* zeta);
+
+/* do_generate(): noise + recurrence.  zeta is a device pointer when
+   zeta_on_device != 0, else a host pointer of nt*nx*ny elements. */
+int arma_hip_ar_generate(arma_hip_ctx* ctx, const arma_ar_desc* desc, void* zeta,
+                         int zeta_on_device);
+
+/* zeta *= sqrt(gamma0 / var(run)), where var is the sample variance of the
+   contiguous run of ceil(nt/2)ceil(nx/2)ceil(ny/2) elements that starts at
+   zeta(nt/2, nx/2, ny/2).  factor_out (optional, host) receives the factor. */
+int arma_hip_rescale(arma_hip_ctx* ctx, void* zeta, const int32_t shape[3], int32_t precision,
+                     double gamma0, double* factor_out);
+
+/* Velocity potential phi(t, z, x, y) of zeta under the linear solver.  Both
+   pointers are device pointers when on_device != 0, else host pointers.
+   wnmax_out (optional, host, 2*npoints[0] doubles) receives per-slice
+   (1/kmin_x, 1/kmin_y).  Returns ARMA_HIP_EINVAL when min z < -depth. */
+int arma_hip_linear_potential(arma_hip_ctx* ctx, const arma_pot_desc* desc, const void* zeta,
+                              void* phi, int on_device, double* wnmax_out);
+
+/* Raw tempered uint32 draws of n_streams streams (draws_per_stream each)
+   into out (device), stream-major: the bit-exact check of the generator. */
+int arma_hip_mt_raw(arma_hip_ctx* ctx, const arma_mt_params* mts, const uint32_t* seeds,
+                    int32_t n_streams, int32_t draws_per_stream, uint32_t* out);
+
+/* Per-kernel timing with HIP events on the context stream.  The report has
+   one line per kernel, "prfl <kernel> <total microseconds>us <launches>", in
+   the style of the reference's profile output (src/profile.hh:70-101). */
+int arma_hip_profile(arma_hip_ctx* ctx, int32_t enable);
+int arma_hip_profile_report(arma_hip_ctx* ctx, char* buf, size_t len, int32_t reset);
+
+/* mt.dat reading and writing (binary mt_struct dumps followed by nn state
+   words; 72-byte header on LP64).  *n_read receives the number of records
+   read; returns ARMA_HIP_EPRNG if fewer than n were present. */
+int arma_hip_read_mt_config(const char* path, int32_t n, arma_mt_params* out,
+                            int32_t* n_read);
+int arma_hip_write_mt_config(const char* path, const arma_mt_params* params, int32_t n);
+
+/* Dynamic creation of one parameter set for Mersenne exponent p and word
+   size w=32, with the 16-bit id embedded in the low bits of aaa, searched
+   from a generator seeded with seed. */
+int arma_hip_dcmt_create(int32_t p, int32_t id, uint32_t seed, arma_mt_params* out);
+/* Creates n parameter sets with ids 0..n-1 using up to nthreads host threads. */
+int arma_hip_dcmt_create_many(int32_t p, int32_t n, uint32_t seed, int32_t nthreads,
+                              arma_mt_params* out);
+
+/* ---- FP64 model setup (host) ---- */
+
+enum arma_acf_function {
+	ARMA_ACF_STANDING_WAVE = 0,     /* standing_wave_ACF, src/acf.hh:18-37 */
+	ARMA_ACF_PROPAGATING_WAVE = 1   /* propagating_wave_ACF, src/acf.hh:39-58 */
+};
+
+/* Closed-form ACF on a grid of shape points with spacing delta
+   (src/acf.hh:18-62 via src/bits/acf_wrapper.cc:41-70). */
+int arma_hip_acf(int32_t func, const double delta[3], const int32_t shape[3], double amplitude,
+                 double velocity, const double alpha[3], const double beta[2], double* out);
+
+/* Choi order-recursive Yule-Walker solver (Yule_walker_solver<T>::solve,
+   src/yule_walker.cc:420-539) on an n^3 autocovariance cube; phi_out holds
+   max_order^3 doubles and receives phi row-major with shape order_out. */
+int arma_hip_yule_walker_choi(const double* acf, int32_t n, int32_t max_order, double var_eps,
+                              int32_t determine_order, int32_t chop, double chop_eps,
+                              double* phi_out, int32_t order_out[3], double* varwn_solver);
+
+/* Gauss-elimination coefficients (AR_model<T>::determine_coefficients_gauss,
+   src/generator/ar_model.cc:97-135); phi_out holds prod(order) doubles. */
+int arma_hip_yule_walker_gauss(const double* acf, const int32_t acf_shape[3],
+                               const int32_t order[3], double* phi_out);
+
+/* AR_model<T>::white_noise_variance (src/generator/ar_model.cc:55-61). */
+double arma_hip_white_noise_variance(const double* acf, const int32_t acf_shape[3],
+                                     const double* phi, const int32_t phi_shape[3]);
+
+/* get_partition_shape (src/generator/ar_model.cc:23-50). */
+void arma_hip_partition_shape(const int32_t partition[3], const int32_t grid[3],
+                              const int32_t order[3], int32_t parallelism, int32_t out[3]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,225 @@
+"""GPU parity of the white noise and the AR recurrence, through the C ABI.
+
+Bars: uint32 streams bit-exact; normals within 4 ulp of the oracle (the
+device log may differ from glibc's by an ulp) with the same accept/reject
+pattern; the plain FP64 recurrence bit-exact; everything else to the
+tolerance written in each test.
+"""
+import numpy as np
+import pytest
+import torch
+
+import arma_amd
+from arma_amd import _abi
+import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+
+def to_oracle(p):
+    return O.MtParams(*p.as_tuple())
+
+
+def ulp_diff(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.max(np.abs(a - b) / np.spacing(np.maximum(np.abs(a), np.abs(b))))
+
+
+def test_mt_raw_bit_exact(hip_ctx, p521_params):
+    plist = [arma_amd.mt19937_params()] + [p521_params[i] for i in range(7)]
+    seeds = np.array([0, 1, 42, 5489, 0, 7, 123456789, 4294967295], np.uint32)
+    draws = 1500
+    out = torch.empty(len(plist) * draws, dtype=torch.int32, device=DEV)
+    hip_ctx.mt_raw(plist, seeds, draws, out)
+    got = out.cpu().numpy().view(np.uint32).reshape(len(plist), draws)
+    for s, p in enumerate(plist):
+        ref = O.mt_fill(to_oracle(p), int(seeds[s]), draws)
+        np.testing.assert_array_equal(got[s], ref)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 1001, 40000])
+def test_sequential_noise_fp64(hip_ctx, n):
+    p = arma_amd.mt19937_params()
+    d = hip_ctx.ar_desc((1, 1, n), np.zeros((1, 1, 1)), 2.5, [p], seeds=[42])
+    eps = torch.empty(n, dtype=torch.float64, device=DEV)
+    hip_ctx.white_noise(d, eps)
+    got = eps.cpu().numpy()
+    ref = O.noise_sequential(to_oracle(p), 42, np.sqrt(2.5), n)
+    u = float(ulp_diff(got, ref))
+    assert u <= 4, u
+    assert float(np.mean(got == ref)) > 0.9
+
+
+def test_sequential_noise_float_semantics(hip_ctx):
+    p = arma_amd.mt19937_params()
+    n = 30000
+    d = hip_ctx.ar_desc((3, 100, 100), np.zeros((1, 1, 1)), 0.37 ** 2, [p], seeds=[1],
+                        precision=arma_amd.FP32, noise_semantics=arma_amd.NOISE_FLOAT)
+    eps = torch.empty(n, dtype=torch.float32, device=DEV)
+    hip_ctx.white_noise(d, eps)
+    got = eps.cpu().numpy()
+    ref = O.noise_sequential(to_oracle(p), 1, np.float32(np.sqrt(np.float32(0.37 ** 2))), n,
+                             dtype=np.float32)
+    ulps = float(np.max(np.abs(got - ref) / np.spacing(np.maximum(np.abs(got), np.abs(ref)))))
+    assert ulps <= 4, ulps
+    # the device logf and glibc's logf round differently in some cases
+    assert float(np.mean(got == ref)) > 0.5
+
+
+def test_partitioned_noise(hip_ctx, p521_params):
+    shape, part = (20, 24, 16), (8, 8, 8)
+    nparts = 3 * 3 * 2
+    plist = [p521_params[i] for i in range(nparts)]
+    seeds = np.arange(nparts, dtype=np.uint32) * 7 + 1
+    d = hip_ctx.ar_desc(shape, np.zeros((1, 1, 1)), 1.7, plist, seeds=seeds,
+                        noise_mode=arma_amd.NOISE_PARTITIONED, partition=part)
+    eps = torch.empty(shape, dtype=torch.float64, device=DEV)
+    hip_ctx.white_noise(d, eps)
+    ref = O.noise_partitioned([to_oracle(p) for p in plist], seeds, shape, part, np.sqrt(1.7))
+    assert ulp_diff(eps.cpu().numpy(), ref) <= 4
+
+
+def test_noise_errors(hip_ctx, p521_params):
+    d = hip_ctx.ar_desc((8, 8, 8), np.zeros((1, 1, 1)), 1.0, [p521_params[0]],
+                        noise_mode=arma_amd.NOISE_PARTITIONED, partition=(4, 4, 4))
+    eps = torch.empty((8, 8, 8), dtype=torch.float64, device=DEV)
+    with pytest.raises(arma_amd.PRNGError):
+        hip_ctx.white_noise(d, eps)
+    d = hip_ctx.ar_desc((8, 8, 8), np.zeros((1, 1, 1)), -1.0, [p521_params[0]])
+    with pytest.raises(ValueError):
+        hip_ctx.white_noise(d, eps)
+
+
+def random_phi(shape, seed, scale=0.04):
+    phi = np.random.default_rng(seed).standard_normal(shape) * scale
+    phi[0, 0, 0] = 0.0
+    return phi
+
+
+@pytest.mark.parametrize("shape,pshape,part", [
+    ((9, 10, 11), (3, 3, 3), (4, 5, 3)),
+    ((16, 12, 20), (2, 3, 4), (16, 12, 20)),
+    ((12, 8, 8), (4, 4, 4), (3, 3, 3)),
+    ((5, 7, 1), (2, 2, 1), (2, 2, 1)),
+])
+def test_plain_recurrence_bit_exact(hip_ctx, shape, pshape, part):
+    rng = np.random.default_rng(11)
+    eps = rng.standard_normal(shape)
+    phi = random_phi(pshape, 3)
+    z = torch.from_numpy(eps.copy()).to(DEV)
+    d = hip_ctx.ar_desc(shape, phi, 1.0, [arma_amd.mt19937_params()], partition=part,
+                        algorithm=arma_amd.AR_PLAIN)
+    hip_ctx.ar_recurrence(d, z)
+    ref = O.ar_generate(eps, phi)
+    np.testing.assert_array_equal(z.cpu().numpy(), ref)
+
+
+def test_plain_rejects_small_partition(hip_ctx):
+    d = hip_ctx.ar_desc((8, 8, 8), random_phi((4, 4, 4), 1), 1.0, [arma_amd.mt19937_params()],
+                        partition=(2, 2, 2))
+    z = torch.zeros((8, 8, 8), dtype=torch.float64, device=DEV)
+    with pytest.raises(ValueError):
+        hip_ctx.ar_recurrence(d, z)
+
+
+def test_rescale_matches_gsl_semantics(hip_ctx):
+    rng = np.random.default_rng(5)
+    for shape in [(5, 6, 7), (64, 33, 17)]:
+        z = rng.standard_normal(shape) * 3 + 1
+        zt = torch.from_numpy(z.copy()).to(DEV)
+        f = hip_ctx.rescale(zt, shape, arma_amd.FP64, 2.0)
+        fref = O.rescale_factor(z, 2.0)
+        assert f == pytest.approx(fref, rel=1e-14)
+        np.testing.assert_allclose(zt.cpu().numpy(), z * f, rtol=0, atol=0)
+        zf = z.astype(np.float32)
+        ztf = torch.from_numpy(zf.copy()).to(DEV)
+        f32 = hip_ctx.rescale(ztf, shape, arma_amd.FP32, 2.0)
+        assert f32 == pytest.approx(O.rescale_factor(zf.astype(np.float64), 2.0), rel=1e-12)
+
+
+def test_surface_pipeline_small(hip_ctx):
+    """do_generate + rescale on the standing-wave configuration at 32x16x16:
+    Choi phi (oracle, FP64 host), sequential noise from MT19937 seed 0."""
+    acf = O.acf(O.STANDING, (10, 10, 10), velocity=0.8, beta=(0.8, 0.8))
+    phi, _ = O.yw_choi(acf, 7)
+    varwn = O.white_noise_variance(acf, phi)
+    shape = (32, 16, 16)
+    p = arma_amd.mt19937_params()
+    d = hip_ctx.ar_desc(shape, phi, varwn, [p], seeds=[0], algorithm=arma_amd.AR_PLAIN,
+                        partition=(16, 16, 16))
+    z = torch.empty(shape, dtype=torch.float64, device=DEV)
+    hip_ctx.ar_generate(d, z)
+    f = hip_ctx.rescale(z, shape, arma_amd.FP64, acf[0, 0, 0])
+    eps = O.noise_sequential(to_oracle(p), 0, np.sqrt(varwn), int(np.prod(shape))).reshape(shape)
+    ref = O.ar_generate(eps, phi)
+    fref = O.rescale_factor(ref, acf[0, 0, 0])
+    assert f == pytest.approx(fref, rel=1e-12)
+    got = z.cpu().numpy()
+    np.testing.assert_allclose(got, ref * fref, rtol=1e-11, atol=1e-12 * np.abs(ref).max())
+    # host-pointer variant of do_generate
+    host = np.empty(shape, np.float64)
+    hip_ctx.ar_generate(d, host, on_device=False)
+    np.testing.assert_allclose(host, ref, rtol=1e-11, atol=1e-12 * np.abs(ref).max())
+
+
+WAVE_CASES = [
+    ((40, 48, 40), (7, 7, 7)),
+    ((33, 17, 50), (7, 7, 2)),
+    ((20, 16, 16), (3, 3, 3)),
+    ((9, 70, 33), (2, 5, 4)),
+    ((1, 31, 20), (1, 3, 3)),
+    ((64, 40, 24), (9, 9, 9)),
+    ((30, 5, 7), (5, 1, 1)),
+]
+
+
+@pytest.mark.parametrize("shape,pshape", WAVE_CASES)
+def test_wavefront_fp64(hip_ctx, shape, pshape):
+    rng = np.random.default_rng(21)
+    eps = rng.standard_normal(shape)
+    phi = random_phi(pshape, 5, scale=0.3 / np.prod(pshape) ** 0.5)
+    z = torch.from_numpy(eps.copy()).to(DEV)
+    d = hip_ctx.ar_desc(shape, phi, 1.0, [arma_amd.mt19937_params()],
+                        algorithm=arma_amd.AR_WAVEFRONT)
+    hip_ctx.ar_recurrence(d, z)
+    ref = O.ar_generate(eps, phi)
+    got = z.cpu().numpy()
+    err = float(np.max(np.abs(got - ref)) / max(1.0, np.max(np.abs(ref))))
+    assert err <= 1e-12, err
+
+
+@pytest.mark.parametrize("shape,pshape", WAVE_CASES[:4])
+def test_wavefront_fp32(hip_ctx, shape, pshape):
+    rng = np.random.default_rng(22)
+    eps = rng.standard_normal(shape)
+    phi = random_phi(pshape, 6, scale=0.3 / np.prod(pshape) ** 0.5)
+    z = torch.from_numpy(eps.astype(np.float32)).to(DEV)
+    d = hip_ctx.ar_desc(shape, phi, 1.0, [arma_amd.mt19937_params()],
+                        algorithm=arma_amd.AR_WAVEFRONT, precision=arma_amd.FP32)
+    hip_ctx.ar_recurrence(d, z)
+    ref = O.ar_generate(eps.astype(np.float32).astype(np.float64), phi)
+    got = z.cpu().numpy().astype(np.float64)
+    rms = float(np.sqrt(np.mean((got - ref) ** 2)) / np.sqrt(np.mean(ref ** 2)))
+    mx = float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))
+    assert rms <= 1e-6 and mx <= 2e-5, (rms, mx)
+
+
+def test_wavefront_standing_wave_surface(hip_ctx):
+    """Choi phi of the standing-wave ACF (7x7x7) on a 96x64x80 surface."""
+    acf = O.acf(O.STANDING, (10, 10, 10), velocity=0.8, beta=(0.8, 0.8))
+    phi, _ = O.yw_choi(acf, 7)
+    varwn = O.white_noise_variance(acf, phi)
+    shape = (96, 64, 80)
+    eps = np.random.default_rng(9).standard_normal(shape) * np.sqrt(varwn)
+    ref = O.ar_generate(eps, phi)
+    for prec, tol in [(arma_amd.FP64, 1e-12), (arma_amd.FP32, 1e-5)]:
+        dt = torch.float64 if prec == arma_amd.FP64 else torch.float32
+        z = torch.from_numpy(eps.copy()).to(DEV, dt)
+        d = hip_ctx.ar_desc(shape, phi, varwn, [arma_amd.mt19937_params()],
+                            algorithm=arma_amd.AR_WAVEFRONT, precision=prec)
+        hip_ctx.ar_recurrence(d, z)
+        got = z.cpu().numpy().astype(np.float64)
+        rms = np.sqrt(np.mean((got - ref) ** 2)) / np.sqrt(np.mean(ref ** 2))
+        assert rms <= tol, (prec, rms)

@@ -1,0 +1,92 @@
+"""GPU parity of the linear velocity potential, through the C ABI.
+
+FP64 bar: max |phi_gpu - phi_oracle| <= 1e-10 * max |phi_oracle| (the GPU
+transform is a half-spectrum C2R of the symmetrised window; the oracle does
+the reference's full complex transforms).  FP32 bar: 2e-4 relative to max.
+The per-slice wave-number range from factor_waves must agree to 1e-15.
+"""
+import numpy as np
+import pytest
+import torch
+
+import arma_amd
+import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+TWO_PI = 2 * np.pi
+
+
+def ar_surface(shape, seed):
+    rng = np.random.default_rng(seed)
+    acf = O.acf(O.PROPAGATING, (10, 10, 10), velocity=0.8, beta=(-0.8, 0.0))
+    phi, _ = O.yw_choi(acf, 4)
+    return O.ar_generate(rng.standard_normal(shape), phi)
+
+
+def run_gpu(ctx, zeta, depth, lb, ub, npts, precision=arma_amd.FP64, want_wn=False):
+    dt = torch.float64 if precision == arma_amd.FP64 else torch.float32
+    z = torch.from_numpy(np.ascontiguousarray(zeta)).to(DEV, dt)
+    out = torch.empty((npts[0], npts[1], zeta.shape[1], zeta.shape[2]), dtype=dt, device=DEV)
+    d = ctx.pot_desc(zeta.shape, (TWO_PI,) * 3, depth, lb, ub, npts, precision)
+    wn = ctx.linear_potential(d, z, out, want_wnmax=want_wn)
+    return out.cpu().numpy().astype(np.float64), wn
+
+
+@pytest.mark.parametrize("shape,npts,lb,ub", [
+    ((6, 16, 16), (6, 4), (0.0, -10.0), (5.0, 0.0)),
+    ((9, 32, 64), (3, 5), (2.0, -8.0), (6.0, 1.0)),
+    ((2, 8, 16), (2, 3), (0.0, -2.0), (1.0, 0.0)),
+    ((12, 64, 32), (12, 16), (0.0, -10.0), (11.0, 0.0)),
+])
+def test_potential_fp64_matches_oracle(hip_ctx, shape, npts, lb, ub):
+    zeta = ar_surface(shape, 1)
+    got, wn = run_gpu(hip_ctx, zeta, 12.0, lb, ub, npts, want_wn=True)
+    ref, wref = O.linear_potential(zeta, (TWO_PI,) * 3, 12.0, lb, ub, npts, return_wnmax=True)
+    np.testing.assert_allclose(wn, wref, rtol=1e-15)
+    assert np.max(np.abs(got - ref)) <= 1e-10 * np.max(np.abs(ref))
+
+
+def test_potential_fp32(hip_ctx):
+    zeta = ar_surface((8, 64, 64), 2)
+    npts, lb, ub = (8, 8), (0.0, -10.0), (7.0, 0.0)
+    got, _ = run_gpu(hip_ctx, zeta, 12.0, lb, ub, npts, precision=arma_amd.FP32)
+    ref = O.linear_potential(zeta, (TWO_PI,) * 3, 12.0, lb, ub, npts)
+    err = np.abs(got - ref)
+    assert np.max(err) <= 2e-4 * np.max(np.abs(ref))
+    assert np.sqrt(np.mean(err ** 2)) <= 1e-5 * np.max(np.abs(ref))
+
+
+def test_potential_single_mode(hip_ctx):
+    nt, nx, ny, p, q = 5, 32, 16, 3, 2
+    x = np.arange(nx)[:, None]
+    y = np.arange(ny)[None, :]
+    mode = np.cos(2 * np.pi * (p * x / nx + q * y / ny)) + 0.3 * np.sin(2 * np.pi * x / nx)
+    zeta = (np.arange(nt)[:, None, None] + 1.0) * mode[None]
+    got, wn = run_gpu(hip_ctx, zeta, 12.0, (2.0, -4.0), (2.0, -1.0), (1, 2), want_wn=True)
+    for j, z in enumerate([-4.0, -1.0]):
+        w = O.window(nx, ny, wn[0, 0], wn[0, 1], z, 12.0)
+
+        def ws(a, b):
+            return 0.5 * (w[a % nx, b % ny] + w[-a % nx, -b % ny])
+        expect = -(np.cos(2 * np.pi * (p * x / nx + q * y / ny)) * ws(p, q) +
+                   0.3 * np.sin(2 * np.pi * x / nx) * ws(1, 0)) / TWO_PI
+        np.testing.assert_allclose(got[0, j][1:, 1:], expect[1:, 1:], rtol=1e-10, atol=1e-11)
+
+
+def test_potential_host_pointers(hip_ctx):
+    zeta = ar_surface((4, 16, 16), 3)
+    npts = (4, 3)
+    d = hip_ctx.pot_desc(zeta.shape, (TWO_PI,) * 3, 12.0, (0.0, -5.0), (3.0, 0.0), npts)
+    out = np.empty((4, 3, 16, 16), np.float64)
+    hip_ctx.linear_potential(d, np.ascontiguousarray(zeta), out, on_device=False)
+    ref = O.linear_potential(zeta, (TWO_PI,) * 3, 12.0, (0.0, -5.0), (3.0, 0.0), npts)
+    assert np.max(np.abs(out - ref)) <= 1e-10 * np.max(np.abs(ref))
+
+
+def test_potential_errors(hip_ctx):
+    zeta = ar_surface((4, 16, 16), 4)
+    with pytest.raises(ValueError):
+        run_gpu(hip_ctx, zeta, 5.0, (0.0, -6.0), (3.0, 0.0), (4, 2))
+    with pytest.raises(ValueError):
+        run_gpu(hip_ctx, ar_surface((4, 12, 16), 4), 12.0, (0.0, -6.0), (3.0, 0.0), (4, 2))
