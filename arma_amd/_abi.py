@@ -33,7 +33,8 @@ EXPORTS = [
     "arma_hip_linear_potential", "arma_hip_mt_raw", "arma_hip_read_mt_config",
     "arma_hip_write_mt_config", "arma_hip_dcmt_create", "arma_hip_dcmt_create_many",
     "arma_hip_acf", "arma_hip_yule_walker_choi", "arma_hip_yule_walker_gauss",
-    "arma_hip_white_noise_variance", "arma_hip_partition_shape",
+    "arma_hip_white_noise_variance", "arma_hip_partition_shape", "arma_hip_profile",
+    "arma_hip_profile_report",
 ]
 
 
@@ -128,6 +129,8 @@ def lib():
         L.arma_hip_white_noise_variance.restype = C.c_double
         L.arma_hip_partition_shape.argtypes = [i3, i3, i3, C.c_int32, i3]
         L.arma_hip_partition_shape.restype = None
+        L.arma_hip_profile.argtypes = [vp, C.c_int32]
+        L.arma_hip_profile_report.argtypes = [vp, C.c_char_p, C.c_size_t, C.c_int32]
         _lib = L
     return _lib
 

@@ -205,21 +205,26 @@ __global__ void mt_raw_kernel(const arma_mt_params* __restrict__ params,
 	for (int k = 0; k < draws; ++k) out[(size_t)s * draws + k] = g.next();
 }
 
-// ---- single stream over the whole grid, one cooperative workgroup ----
+// ---- one stream per workgroup: cooperative twist, parallel polar attempts ----
+//
+// A workgroup owns one stream and the block of the grid it fills (the whole
+// grid for framework=none, one partition otherwise).  The twist runs in
+// chunks of c = nn - mm words (within a chunk every new word depends only on
+// old words and on earlier chunks); each round evaluates THREADS polar
+// attempts in parallel, compacts the accepted ones with a ballot prefix sum
+// and writes their two variates at the positions the sequential generator
+// would have given them.
 
-constexpr int kSeqThreads = 256;
-constexpr int kSeqBuf = 4096 + kMaxNN;
-
-// Chunked parallel twist: within a chunk of c = nn - mm words every new word
-// depends only on old words and on words of earlier chunks.
+template <int THREADS, int MAXNN>
 __device__ void twist_parallel(uint32_t* st, const arma_mt_params& p, int tid) {
+	constexpr int PER = (MAXNN + THREADS - 1) / THREADS;
 	const int n = p.nn, m = p.mm, c = n - m;
 	for (int k0 = 0; k0 < n; k0 += c) {
 		const int k1 = min(k0 + c, n);
-		uint32_t v[8];
+		uint32_t v[PER];
 #pragma unroll
-		for (int j = 0; j < 8; ++j) {
-			const int k = k0 + tid + j * kSeqThreads;
+		for (int j = 0; j < PER; ++j) {
+			const int k = k0 + tid + j * THREADS;
 			if (k < k1) {
 				if (k < n - m) v[j] = mt_twist_word(st[k], st[k + 1], st[k + m], p);
 				else if (k < n - 1) v[j] = mt_twist_word(st[k], st[k + 1], st[k + m - n], p);
@@ -228,39 +233,56 @@ __device__ void twist_parallel(uint32_t* st, const arma_mt_params& p, int tid) {
 		}
 		__syncthreads();
 #pragma unroll
-		for (int j = 0; j < 8; ++j) {
-			const int k = k0 + tid + j * kSeqThreads;
+		for (int j = 0; j < PER; ++j) {
+			const int k = k0 + tid + j * THREADS;
 			if (k < k1) st[k] = v[j];
 		}
 		__syncthreads();
 	}
 }
 
-template <class OutT, int SEM>
-__global__ __launch_bounds__(kSeqThreads) void noise_sequential_kernel(
-	arma_mt_params p, uint32_t seed, double stddev, float stddev_f, OutT* __restrict__ out,
-	long long n) {
-	__shared__ uint32_t st[kMaxNN];
-	__shared__ uint32_t buf[kSeqBuf];
-	__shared__ int wave_tot[kSeqThreads / 64];
+template <class OutT, int SEM, int THREADS, int MAXNN>
+__global__ __launch_bounds__(THREADS) void noise_stream_kernel(
+	const arma_mt_params* __restrict__ params, const uint32_t* __restrict__ seeds, int3 shape,
+	int3 part, int3 np, double stddev, float stddev_f, OutT* __restrict__ out) {
+	constexpr int words_per_attempt = SEM == ARMA_NOISE_DOUBLE ? 4 : 2;
+	constexpr int round_words = THREADS * words_per_attempt;
+	constexpr int KEEP = (MAXNN + THREADS - 1) / THREADS;
+	__shared__ uint32_t st[MAXNN];
+	__shared__ uint32_t buf[round_words + MAXNN];
+	__shared__ int wave_tot[THREADS / 64];
 	const int tid = threadIdx.x;
 	const int lane = tid & 63, wave = tid >> 6;
+	const int pid = blockIdx.x;
+	const arma_mt_params p = params[pid];
+	// the block of the grid this stream fills, row-major
+	const int a = pid / (np.y * np.z), b = (pid / np.z) % np.y, c = pid % np.z;
+	const int lo0 = a * part.x, lo1 = b * part.y, lo2 = c * part.z;
+	const int e1 = min(lo1 + part.y, shape.y) - lo1;
+	const int e2 = min(lo2 + part.z, shape.z) - lo2;
+	const long long e12 = (long long)e1 * e2;
+	const long long n = (long long)(min(lo0 + part.x, shape.x) - lo0) * e12;
+	auto put = [&](long long q, OutT v) {
+		const long long t = q / e12;
+		const long long r = q - t * e12;
+		const int x = int(r / e2);
+		const int y = int(r - (long long)x * e2);
+		out[(size_t(lo0 + t) * shape.y + (lo1 + x)) * shape.z + (lo2 + y)] = v;
+	};
 	if (tid == 0) {
-		uint32_t s = seed;
+		uint32_t s = seeds ? seeds[pid] : 0u;
 		for (int k = 0; k < p.nn; ++k) {
 			st[k] = s & p.wmask;
 			s = 1812433253u * (s ^ (s >> 30)) + uint32_t(k + 1);
 		}
 	}
 	__syncthreads();
-	constexpr int words_per_attempt = SEM == ARMA_NOISE_DOUBLE ? 4 : 2;
-	constexpr int round_words = kSeqThreads * words_per_attempt;
 	int avail = 0;
 	long long produced = 0;
 	while (produced < n) {
 		while (avail < round_words) {
-			twist_parallel(st, p, tid);
-			for (int k = tid; k < p.nn; k += kSeqThreads) buf[avail + k] = mt_temper(st[k], p);
+			twist_parallel<THREADS, MAXNN>(st, p, tid);
+			for (int k = tid; k < p.nn; k += THREADS) buf[avail + k] = mt_temper(st[k], p);
 			avail += p.nn;
 			__syncthreads();
 		}
@@ -284,7 +306,7 @@ __global__ __launch_bounds__(kSeqThreads) void noise_sequential_kernel(
 		if (lane == 0) wave_tot[wave] = __popcll(mask);
 		__syncthreads();
 		int before = 0, total = 0;
-		for (int q = 0; q < kSeqThreads / 64; ++q) {
+		for (int q = 0; q < THREADS / 64; ++q) {
 			if (q < wave) before += wave_tot[q];
 			total += wave_tot[q];
 		}
@@ -293,12 +315,12 @@ __global__ __launch_bounds__(kSeqThreads) void noise_sequential_kernel(
 			if (pos < n) {
 				if (SEM == ARMA_NOISE_DOUBLE) {
 					const double mult = sqrt(-2.0 * log(r2d) / r2d);
-					out[pos] = OutT(yd * mult * stddev + 0.0);
-					if (pos + 1 < n) out[pos + 1] = OutT(xd * mult * stddev + 0.0);
+					put(pos, OutT(yd * mult * stddev + 0.0));
+					if (pos + 1 < n) put(pos + 1, OutT(xd * mult * stddev + 0.0));
 				} else {
 					const float mult = sqrtf(-2.0f * logf(r2f) / r2f);
-					out[pos] = OutT(yf * mult * stddev_f + 0.0f);
-					if (pos + 1 < n) out[pos + 1] = OutT(xf * mult * stddev_f + 0.0f);
+					put(pos, OutT(yf * mult * stddev_f + 0.0f));
+					if (pos + 1 < n) put(pos + 1, OutT(xf * mult * stddev_f + 0.0f));
 				}
 			}
 		}
@@ -306,16 +328,16 @@ __global__ __launch_bounds__(kSeqThreads) void noise_sequential_kernel(
 		__syncthreads();
 		// drop the consumed words
 		const int left = avail - round_words;
-		uint32_t keep[(kMaxNN + kSeqThreads - 1) / kSeqThreads];
+		uint32_t keep[KEEP];
 #pragma unroll
-		for (int j = 0; j < (kMaxNN + kSeqThreads - 1) / kSeqThreads; ++j) {
-			const int k = tid + j * kSeqThreads;
+		for (int j = 0; j < KEEP; ++j) {
+			const int k = tid + j * THREADS;
 			if (k < left) keep[j] = buf[round_words + k];
 		}
 		__syncthreads();
 #pragma unroll
-		for (int j = 0; j < (kMaxNN + kSeqThreads - 1) / kSeqThreads; ++j) {
-			const int k = tid + j * kSeqThreads;
+		for (int j = 0; j < KEEP; ++j) {
+			const int k = tid + j * THREADS;
 			if (k < left) buf[k] = keep[j];
 		}
 		avail = left;
@@ -355,24 +377,27 @@ void launch_white_noise(arma_hip_ctx* ctx, const arma_ar_desc* d, void* eps) {
 	if (d->noise_mode == ARMA_NOISE_SEQUENTIAL) {
 		ARMA_HIP_REQUIRE(d->n_mts >= 1 && d->mts, ARMA_HIP_EPRNG, "bad number of MT configs");
 		validate_params(d->mts, 1);
-		const long long n = (long long)numel3(d->shape);
-		const uint32_t seed = d->seeds ? d->seeds[0] : 0u;
-		KernelTimer timer(ctx, "noise_sequential");
-		if (fp32) {
-			if (sem == ARMA_NOISE_DOUBLE)
-				noise_sequential_kernel<float, ARMA_NOISE_DOUBLE><<<1, kSeqThreads, 0, ctx->stream>>>(
-					d->mts[0], seed, stddev, stddev_f, (float*)eps, n);
-			else
-				noise_sequential_kernel<float, ARMA_NOISE_FLOAT><<<1, kSeqThreads, 0, ctx->stream>>>(
-					d->mts[0], seed, stddev, stddev_f, (float*)eps, n);
-		} else {
-			if (sem == ARMA_NOISE_DOUBLE)
-				noise_sequential_kernel<double, ARMA_NOISE_DOUBLE><<<1, kSeqThreads, 0, ctx->stream>>>(
-					d->mts[0], seed, stddev, stddev_f, (double*)eps, n);
-			else
-				noise_sequential_kernel<double, ARMA_NOISE_FLOAT><<<1, kSeqThreads, 0, ctx->stream>>>(
-					d->mts[0], seed, stddev, stddev_f, (double*)eps, n);
+		auto* dp = (arma_mt_params*)ctx->params.get(sizeof(arma_mt_params));
+		upload(ctx, dp, d->mts, sizeof(arma_mt_params));
+		uint32_t* ds = nullptr;
+		if (d->seeds) {
+			ds = (uint32_t*)ctx->seeds.get(sizeof(uint32_t));
+			upload(ctx, ds, d->seeds, sizeof(uint32_t));
 		}
+		const int3 shape{d->shape[0], d->shape[1], d->shape[2]};
+		const int3 one{1, 1, 1};
+		KernelTimer timer(ctx, "noise_sequential");
+#define ARMA_LAUNCH_SEQ(OT, S)                                                                   \
+	noise_stream_kernel<OT, S, 256, kMaxNN><<<1, 256, 0, ctx->stream>>>(dp, ds, shape, shape, one, \
+	                                                                   stddev, stddev_f, (OT*)eps)
+		if (fp32) {
+			if (sem == ARMA_NOISE_DOUBLE) ARMA_LAUNCH_SEQ(float, ARMA_NOISE_DOUBLE);
+			else ARMA_LAUNCH_SEQ(float, ARMA_NOISE_FLOAT);
+		} else {
+			if (sem == ARMA_NOISE_DOUBLE) ARMA_LAUNCH_SEQ(double, ARMA_NOISE_DOUBLE);
+			else ARMA_LAUNCH_SEQ(double, ARMA_NOISE_FLOAT);
+		}
+#undef ARMA_LAUNCH_SEQ
 		ARMA_HIP_CHECK(hipGetLastError());
 		return;
 	}
@@ -394,10 +419,28 @@ void launch_white_noise(arma_hip_ctx* ctx, const arma_ar_desc* d, void* eps) {
 		dseeds = (uint32_t*)ctx->seeds.get(sizeof(uint32_t) * ntotal);
 		upload(ctx, dseeds, d->seeds, sizeof(uint32_t) * ntotal);
 	}
+	KernelTimer timer(ctx, "noise_partitioned");
+	if (nn_max <= 64) {
+		// one wave per stream (dcmt exponents up to 1279)
+#define ARMA_LAUNCH_WAVE(OT, S)                                                                  \
+	noise_stream_kernel<OT, S, 64, 64><<<ntotal, 64, 0, ctx->stream>>>(dparams, dseeds, shape,   \
+	                                                                  part, np, stddev,         \
+	                                                                  stddev_f, (OT*)eps)
+		if (fp32) {
+			if (sem == ARMA_NOISE_DOUBLE) ARMA_LAUNCH_WAVE(float, ARMA_NOISE_DOUBLE);
+			else ARMA_LAUNCH_WAVE(float, ARMA_NOISE_FLOAT);
+		} else {
+			if (sem == ARMA_NOISE_DOUBLE) ARMA_LAUNCH_WAVE(double, ARMA_NOISE_DOUBLE);
+			else ARMA_LAUNCH_WAVE(double, ARMA_NOISE_FLOAT);
+		}
+#undef ARMA_LAUNCH_WAVE
+		ARMA_HIP_CHECK(hipGetLastError());
+		return;
+	}
+	// large states: one thread per stream with the state in LDS
 	const int tpb = std::min(threads_for_state(nn_max), 64);
 	const int blocks = div_ceil(ntotal, tpb);
 	const size_t lds = size_t(tpb) * nn_max * 4;
-	KernelTimer timer(ctx, "noise_partitioned");
 #define ARMA_LAUNCH_PART(OT, S)                                                                  \
 	noise_partitioned_kernel<OT, S><<<blocks, tpb, lds, ctx->stream>>>(dparams, dseeds, shape,   \
 	                                                                  part, np, stddev,         \

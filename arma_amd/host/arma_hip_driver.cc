@@ -33,8 +33,14 @@ void check(int rc) {
 	}
 }
 
-Hip_context::Hip_context(int device) { check(arma_hip_init(device, &ctx_)); }
-Hip_context::~Hip_context() { arma_hip_destroy(ctx_); }
+arma_hip_ctx* Hip_context::get() {
+	if (!ctx_) check(arma_hip_init(device_, &ctx_));
+	return ctx_;
+}
+
+Hip_context::~Hip_context() {
+	if (ctx_) arma_hip_destroy(ctx_);
+}
 
 // ---- input syntax (src/params.cc:9-57 and the operator>> of each type) ----
 
@@ -311,8 +317,8 @@ Array3D<T> AR_model<T>::do_generate() {
 }
 
 template <class T>
-Array3D<T> AR_model<T>::generate() {
-	// src/generator/basic_arma_model.cc:51-113 with the closed-form ACF
+void AR_model<T>::prepare() {
+	// src/generator/basic_arma_model.cc:51-97 with the closed-form ACF
 	if (!(order_[0] > 0 && order_[1] > 0 && order_[2] > 0)) order_ = acf_shape_;
 	for (int d = 0; d < 3; ++d)
 		this->outgrid_.length[d] = acf_grid_.delta(d) * this->outgrid_.num_patches(d) * 2.0 * M_PI;
@@ -320,6 +326,24 @@ Array3D<T> AR_model<T>::generate() {
 	std::clog << "White noise variance = " << varwn_ << '\n';
 	if (validate_)
 		std::clog << "validate: the process root check needs GSL and is not part of the HIP build\n";
+}
+
+template <class T>
+void AR_model<T>::describe(std::ostream& out) const {
+	out.precision(17);
+	out << "grid=(" << this->outgrid_.npoints[0] << ',' << this->outgrid_.npoints[1] << ','
+	    << this->outgrid_.npoints[2] << "):(" << this->outgrid_.length[0] << ','
+	    << this->outgrid_.length[1] << ',' << this->outgrid_.length[2] << ")\n";
+	out << "order=(" << order_[0] << ',' << order_[1] << ',' << order_[2] << ")\n";
+	out << "varwn=" << varwn_ << "\n";
+	out << "phi=";
+	for (size_t i = 0; i < phi_.size(); ++i) out << (i ? "," : "") << phi_[i];
+	out << "\n";
+}
+
+template <class T>
+Array3D<T> AR_model<T>::generate() {
+	prepare();
 	return do_generate();
 }
 
@@ -402,6 +426,15 @@ void ARMA_driver<T>::read(std::istream& in) {
 	params.read(in);
 	if (!solver_) throw std::runtime_error("bad solver");
 	if (!model_) throw std::runtime_error("bad generator");
+}
+
+template <class T>
+void ARMA_driver<T>::dry_run(std::ostream& out) {
+	model_->prepare();
+	model_->describe(out);
+	const auto& d = solver_->domain();
+	out << "domain=from (" << d.lbound[0] << ',' << d.lbound[1] << ") to (" << d.ubound[0] << ','
+	    << d.ubound[1] << ") npoints (" << d.npoints[0] << ',' << d.npoints[1] << ")\n";
 }
 
 template <class T>

@@ -95,16 +95,18 @@ private:
 
 enum class Precision { FP64, FP32 };
 
-// Device and stream shared by the models and solvers of one driver.
+// Device and stream shared by the models and solvers of one driver; the
+// device is opened on first use, so parsing and model setup need no GPU.
 class Hip_context {
 public:
-	explicit Hip_context(int device = 0);
+	explicit Hip_context(int device = 0) : device_(device) {}
 	~Hip_context();
 	Hip_context(const Hip_context&) = delete;
 	Hip_context& operator=(const Hip_context&) = delete;
-	arma_hip_ctx* get() const { return ctx_; }
+	arma_hip_ctx* get();
 
 private:
+	int device_;
 	arma_hip_ctx* ctx_ = nullptr;
 };
 
@@ -119,6 +121,9 @@ public:
 	virtual ~Basic_model() = default;
 	virtual void read(std::istream& in) = 0;
 	virtual Array3D<T> generate() = 0;
+	// model setup without the GPU part (ACF, coefficients), for dry runs
+	virtual void prepare() = 0;
+	virtual void describe(std::ostream& out) const = 0;
 	const Grid<3>& grid() const { return outgrid_; }
 	Output_flags oflags() const { return oflags_; }
 
@@ -137,6 +142,8 @@ public:
 	explicit AR_model(std::shared_ptr<Hip_context> ctx) : ctx_(std::move(ctx)) {}
 	void read(std::istream& in) override;
 	Array3D<T> generate() override;
+	void prepare() override;
+	void describe(std::ostream& out) const override;
 	const std::vector<double>& coefficients() const { return phi_; }
 	Shape3D order() const { return order_; }
 	double white_noise_variance() const { return varwn_; }
@@ -210,6 +217,8 @@ public:
 
 	void open(const std::string& filename);
 	void read(std::istream& in);
+	// parse + model setup only; prints the model (no GPU needed)
+	void dry_run(std::ostream& out);
 	void generate_wavy_surface();
 	void compute_velocity_potentials();
 	void write_all();

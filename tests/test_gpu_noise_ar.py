@@ -223,3 +223,17 @@ def test_wavefront_standing_wave_surface(hip_ctx):
         got = z.cpu().numpy().astype(np.float64)
         rms = np.sqrt(np.mean((got - ref) ** 2)) / np.sqrt(np.mean(ref ** 2))
         assert rms <= tol, (prec, rms)
+
+
+def test_partitioned_noise_large_state(hip_ctx):
+    """MT19937-sized records (624 words) take the thread-per-stream kernel."""
+    shape, part = (6, 10, 9), (3, 5, 9)
+    plist = [arma_amd.mt19937_params()] * 4
+    seeds = np.array([5, 6, 7, 8], np.uint32)
+    d = hip_ctx.ar_desc(shape, np.zeros((1, 1, 1)), 0.5, plist, seeds=seeds,
+                        noise_mode=arma_amd.NOISE_PARTITIONED, partition=part)
+    eps = torch.empty(shape, dtype=torch.float64, device=DEV)
+    hip_ctx.white_noise(d, eps)
+    ref = O.noise_partitioned([to_oracle(p) for p in plist], seeds, shape, part, np.sqrt(0.5))
+    u = float(ulp_diff(eps.cpu().numpy(), ref))
+    assert u <= 4, u
