@@ -166,6 +166,25 @@ def dcmt_create_many(p, n, seed, nthreads=None):
     return arr
 
 
+DCMT_TABLE = os.path.join(HERE, "data", "dcmt_p521_aaa_seed20240611.npy")
+DCMT_TABLE_SEED = 20240611
+
+
+def dcmt_p521_params(n, nthreads=None):
+    """n dynamic-creator records for p = 521 (ids 0..n-1, creator seed
+    20240611).  The first 4096 come from the shipped table of aaa values
+    (every other field is fixed for p = 521, w = 32); more are created."""
+    table = np.load(DCMT_TABLE) if os.path.exists(DCMT_TABLE) else np.zeros(0, np.uint32)
+    if n > table.size:
+        return dcmt_create_many(521, n, DCMT_TABLE_SEED, nthreads)
+    tmpl = dcmt_create(521, 0, DCMT_TABLE_SEED)
+    arr = (MtParams * n)()
+    for i in range(n):
+        arr[i] = tmpl
+        arr[i].aaa = int(table[i])
+    return arr
+
+
 def read_mt_config(path, n):
     arr = (MtParams * n)()
     got = C.c_int32()

@@ -55,38 +55,81 @@ int ilog2_exact(int n) {
 
 __global__ void twiddle_kernel(double2* tw, int n) {
 	const int q = blockIdx.x * blockDim.x + threadIdx.x;
-	if (q < n / 2) {
+	if (q < n) {
 		double s, c;
 		sincospi(2.0 * double(q) / double(n), &s, &c);
 		tw[q] = make_double2(c, -s);  // e^{-2 pi i q / n}
 	}
 }
 
-// Radix-2 Stockham transforms of nb sequences of length n (n = 2^log2n)
-// held in LDS.  sign < 0: forward e^{-i}, sign > 0: backward e^{+i}
-// (unnormalised), as gsl_fft_complex_transform.  Returns the buffer that
-// holds the result.
+template <class R>
+__device__ __forceinline__ cpx<R> twiddled(const cpx<R>& a, const double2* __restrict__ tw, int idx,
+                                           int sign) {
+	const double2 w = tw[idx];
+	const R wr = R(w.x);
+	const R wi = sign < 0 ? R(w.y) : R(-w.y);
+	return cpx<R>{a.x * wr - a.y * wi, a.x * wi + a.y * wr};
+}
+
+// Stockham transforms (radix-4 passes, one radix-2 pass when log2 n is odd)
+// of nb sequences of length n = 2^log2n held in LDS.  sign < 0: forward
+// e^{-i}, sign > 0: backward e^{+i} (unnormalised), as
+// gsl_fft_complex_transform.  tw[q] = e^{-2 pi i q / n}, q < n.  Returns the
+// buffer that holds the result.
 template <class R>
 __device__ cpx<R>* fft_lds(cpx<R>* x, cpx<R>* y, int n, int log2n, int nb,
                            const double2* __restrict__ tw, int sign) {
-	const int half = n >> 1;
-	const int total = nb * half;
-	for (int s = 0; s < log2n; ++s) {
-		const int Ns = 1 << s;
+	int ls = 0;  // log2 of the current sub-transform length Ns
+	while (log2n - ls >= 2) {
+		const int Ns = 1 << ls;
+		const int quarter = n >> 2;
+		const int total = nb * quarter;
+		const int tstep = log2n - 2 - ls;  // twiddle stride n / (4 Ns)
+		for (int q = threadIdx.x; q < total; q += blockDim.x) {
+			const int b = q >> (log2n - 2);
+			const int j = q & (quarter - 1);
+			const cpx<R>* src = x + b * n + j;
+			cpx<R> a0 = src[0];
+			cpx<R> a1 = src[quarter];
+			cpx<R> a2 = src[2 * quarter];
+			cpx<R> a3 = src[3 * quarter];
+			const int k = j & (Ns - 1);
+			if (k) {
+				a1 = twiddled(a1, tw, k << tstep, sign);
+				a2 = twiddled(a2, tw, (2 * k) << tstep, sign);
+				a3 = twiddled(a3, tw, (3 * k) << tstep, sign);
+			}
+			const cpx<R> b0{a0.x + a2.x, a0.y + a2.y};
+			const cpx<R> b1{a0.x - a2.x, a0.y - a2.y};
+			const cpx<R> b2{a1.x + a3.x, a1.y + a3.y};
+			// (a1 - a3) times -i (forward) or +i (backward)
+			const cpx<R> d13{a1.x - a3.x, a1.y - a3.y};
+			const cpx<R> b3 = sign < 0 ? cpx<R>{d13.y, -d13.x} : cpx<R>{-d13.y, d13.x};
+			cpx<R>* dst = y + b * n + ((j >> ls) << (ls + 2)) + k;
+			dst[0] = cpx<R>{b0.x + b2.x, b0.y + b2.y};
+			dst[Ns] = cpx<R>{b1.x + b3.x, b1.y + b3.y};
+			dst[2 * Ns] = cpx<R>{b0.x - b2.x, b0.y - b2.y};
+			dst[3 * Ns] = cpx<R>{b1.x - b3.x, b1.y - b3.y};
+		}
+		__syncthreads();
+		cpx<R>* t = x;
+		x = y;
+		y = t;
+		ls += 2;
+	}
+	if (ls < log2n) {
+		const int Ns = 1 << ls;
+		const int half = n >> 1;
+		const int total = nb * half;
 		for (int q = threadIdx.x; q < total; q += blockDim.x) {
 			const int b = q >> (log2n - 1);
 			const int j = q & (half - 1);
 			const cpx<R> a0 = x[b * n + j];
-			const cpx<R> a1 = x[b * n + j + half];
 			const int k = j & (Ns - 1);
-			const double2 w = tw[k << (log2n - 1 - s)];
-			const R wr = R(w.x);
-			const R wi = sign < 0 ? R(w.y) : R(-w.y);
-			const R tr = a1.x * wr - a1.y * wi;
-			const R ti = a1.x * wi + a1.y * wr;
-			const int d = ((j >> s) << (s + 1)) + k;
-			y[b * n + d] = cpx<R>{a0.x + tr, a0.y + ti};
-			y[b * n + d + Ns] = cpx<R>{a0.x - tr, a0.y - ti};
+			const cpx<R> a1 = twiddled(x[b * n + j + half], tw, k, sign);
+			const int d = ((j >> ls) << (ls + 1)) + k;
+			y[b * n + d] = cpx<R>{a0.x + a1.x, a0.y + a1.y};
+			y[b * n + d + Ns] = cpx<R>{a0.x - a1.x, a0.y - a1.y};
 		}
 		__syncthreads();
 		cpx<R>* t = x;
@@ -542,10 +585,10 @@ void run_potential(arma_hip_ctx* ctx, const arma_pot_desc* d, const T* zeta, T* 
 	}
 	const hipStream_t st = ctx->stream;
 	// twiddles (FP64, generated on the device)
-	double2* twx = (double2*)ctx->pot_twiddle.get(sizeof(double2) * size_t(nx / 2 + ny / 2));
-	double2* twy = twx + nx / 2;
-	twiddle_kernel<<<div_ceil(nx / 2, 256), 256, 0, st>>>(twx, nx);
-	twiddle_kernel<<<div_ceil(ny / 2, 256), 256, 0, st>>>(twy, ny);
+	double2* twx = (double2*)ctx->pot_twiddle.get(sizeof(double2) * size_t(nx + ny));
+	double2* twy = twx + nx;
+	twiddle_kernel<<<div_ceil(nx, 256), 256, 0, st>>>(twx, nx);
+	twiddle_kernel<<<div_ceil(ny, 256), 256, 0, st>>>(twy, ny);
 	int* didx = (int*)ctx->pot_wn.get(sizeof(int) * 2 * size_t(nto));
 	int* dtint = didx + nto;
 	std::vector<int> both(idx);

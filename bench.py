@@ -40,7 +40,7 @@ WORKLOADS = {
     # precision variant: FP32 storage, compensated accumulation, FP64 setup
     "ar-potential-2048x512x512-fp32": dict(
         acf=("propagating", 0.8, (-0.8, 0.0)), shape=(2048, 512, 512), order=7,
-        partition=(16, 32, 32), seed=1, depth=12.0, lbound=(0.0, -10.0),
+        partition=(32, 64, 64), seed=1, depth=12.0, lbound=(0.0, -10.0),
         ubound=(2047.0, 0.0), npoints=(2048, 32), precision="fp32"),
     # CPU-runnable plain AR case (no potential)
     "ar-standing-128x64x64-fp64": dict(
@@ -146,8 +146,8 @@ def main():
     part = w["partition"]
     nstreams = math.prod(math.ceil(w["shape"][d] / part[d]) for d in range(3))
     t0 = time.time()
-    params = arma_amd.dcmt_create_many(521, nstreams, 20240611, min(16, os.cpu_count() or 1))
-    log(f"bench: created {nstreams} MT parameter sets in {time.time() - t0:.1f}s")
+    params = arma_amd._abi.dcmt_p521_params(nstreams, min(16, os.cpu_count() or 1))
+    log(f"bench: {nstreams} MT parameter sets ready in {time.time() - t0:.1f}s")
     seeds = np.full(nstreams, w["seed"], np.uint32)
 
     ctx = arma_amd.Context(0)

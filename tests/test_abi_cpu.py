@@ -123,3 +123,21 @@ def test_device_entry_points_fail_cleanly_without_context():
     rc = L.arma_hip_synchronize(None)
     assert rc == _abi.ARMA_HIP_EINVAL
     assert b"null context" in L.arma_hip_last_error()
+
+
+def test_shipped_dcmt_table():
+    """The shipped aaa table is what the creator makes for ids 0..4095 with
+    the table seed, and its records are primitive."""
+    tab = arma_amd._abi.dcmt_p521_params(4096)
+    for i in (0, 1, 777, 4095):
+        again = arma_amd.dcmt_create(521, i, arma_amd._abi.DCMT_TABLE_SEED)
+        assert tab[i].as_tuple() == again.as_tuple()
+        assert (tab[i].aaa & 0xFFFF) == i
+    p = tab[2024]
+    bits = [int(v) & 1 for v in O.mt_fill(O.MtParams(*p.as_tuple()), 1, 2 * 521 + 16)]
+    conn, L = gf2_berlekamp_massey(bits)
+    assert L == 521
+    x = 2
+    for _ in range(521):
+        x = gf2_mulmod(x, x, conn, 521)
+    assert x == 2
