@@ -12,9 +12,11 @@
 //   W_rot    = W cyclically half-shifted        (src/blitz.hh:54-71).
 //
 // GPU formulation:
-//  * zeta_t is real, so FFT2(zeta_t) is Hermitian and
+//  * zeta_t is real, so F = FFT2(zeta_t) is Hermitian and
 //    Re(IFFT2(F W_rot)) = C2R(F W_sym) with W_sym(k) = (W_rot(k) + W_rot(-k))/2.
-//    Only the half spectrum ky = 0..ny/2 is transformed and stored.
+//    Only the half spectrum ky = 0..M (M = ny/2) is formed.  After the
+//    inverse along x the columns ky = 0 and ky = M are real; they travel
+//    packed as one complex column, so every slice is M complex columns.
 //  * The forward spectrum of a slice is computed once and reused for every z
 //    (the reference recomputes it per z).
 //  * W is evaluated as 4 pi (e^{lz} + e^{-l(z+2h)}) / (1 + e^{-2lh}), which
@@ -23,27 +25,30 @@
 //    e^{l dz}.  Where the reference's expression overflows to NaN (and is
 //    then zeroed) the true W is below 4 pi e^{-354}; this formulation returns
 //    that tiny value instead of 0.
-//  * 1-D transforms are radix-2 Stockham passes in LDS over batches of rows
-//    or columns; twiddles are generated in FP64 on the GPU and rounded to the
-//    working precision.
+//  * Every 1-D transform is a four-step transform of register-resident
+//    short DFTs with one LDS exchange (fft_reg.hpp).  Rows of length ny are
+//    real: they go through a complex transform of length M with the usual
+//    even/odd split.  Twiddles are generated in FP64 on the GPU and rounded
+//    to the working precision.
+//  * Intermediate layouts are column-major ([column][x]) so that both the
+//    x-transforms and the row transforms read and write whole cache lines.
 //  * factor_waves is a sequential run scan per slice; it runs as a chunked
 //    scan where each thread summarises a contiguous chunk of events and one
 //    thread folds the summaries in order (the fold is exact: representatives
 //    are first-maximum picks, pairs are scored only between adjacent closed
 //    runs).
 #include "common.hpp"
+#include "fft_reg.hpp"
 
 #include <cmath>
+#include <cstdlib>
+#include <utility>
 #include <vector>
 
 namespace arma_hip {
 namespace {
 
-template <class R>
-struct cpx {
-	R x, y;
-};
-
+constexpr int kThreads = 256;
 constexpr double kTwoPi = 6.283185307179586476925286766559;
 constexpr double kFourPi = 12.566370614359172953850573533118;
 
@@ -60,83 +65,6 @@ __global__ void twiddle_kernel(double2* tw, int n) {
 		sincospi(2.0 * double(q) / double(n), &s, &c);
 		tw[q] = make_double2(c, -s);  // e^{-2 pi i q / n}
 	}
-}
-
-template <class R>
-__device__ __forceinline__ cpx<R> twiddled(const cpx<R>& a, const double2* __restrict__ tw, int idx,
-                                           int sign) {
-	const double2 w = tw[idx];
-	const R wr = R(w.x);
-	const R wi = sign < 0 ? R(w.y) : R(-w.y);
-	return cpx<R>{a.x * wr - a.y * wi, a.x * wi + a.y * wr};
-}
-
-// Stockham transforms (radix-4 passes, one radix-2 pass when log2 n is odd)
-// of nb sequences of length n = 2^log2n held in LDS.  sign < 0: forward
-// e^{-i}, sign > 0: backward e^{+i} (unnormalised), as
-// gsl_fft_complex_transform.  tw[q] = e^{-2 pi i q / n}, q < n.  Returns the
-// buffer that holds the result.
-template <class R>
-__device__ cpx<R>* fft_lds(cpx<R>* x, cpx<R>* y, int n, int log2n, int nb,
-                           const double2* __restrict__ tw, int sign) {
-	int ls = 0;  // log2 of the current sub-transform length Ns
-	while (log2n - ls >= 2) {
-		const int Ns = 1 << ls;
-		const int quarter = n >> 2;
-		const int total = nb * quarter;
-		const int tstep = log2n - 2 - ls;  // twiddle stride n / (4 Ns)
-		for (int q = threadIdx.x; q < total; q += blockDim.x) {
-			const int b = q >> (log2n - 2);
-			const int j = q & (quarter - 1);
-			const cpx<R>* src = x + b * n + j;
-			cpx<R> a0 = src[0];
-			cpx<R> a1 = src[quarter];
-			cpx<R> a2 = src[2 * quarter];
-			cpx<R> a3 = src[3 * quarter];
-			const int k = j & (Ns - 1);
-			if (k) {
-				a1 = twiddled(a1, tw, k << tstep, sign);
-				a2 = twiddled(a2, tw, (2 * k) << tstep, sign);
-				a3 = twiddled(a3, tw, (3 * k) << tstep, sign);
-			}
-			const cpx<R> b0{a0.x + a2.x, a0.y + a2.y};
-			const cpx<R> b1{a0.x - a2.x, a0.y - a2.y};
-			const cpx<R> b2{a1.x + a3.x, a1.y + a3.y};
-			// (a1 - a3) times -i (forward) or +i (backward)
-			const cpx<R> d13{a1.x - a3.x, a1.y - a3.y};
-			const cpx<R> b3 = sign < 0 ? cpx<R>{d13.y, -d13.x} : cpx<R>{-d13.y, d13.x};
-			cpx<R>* dst = y + b * n + ((j >> ls) << (ls + 2)) + k;
-			dst[0] = cpx<R>{b0.x + b2.x, b0.y + b2.y};
-			dst[Ns] = cpx<R>{b1.x + b3.x, b1.y + b3.y};
-			dst[2 * Ns] = cpx<R>{b0.x - b2.x, b0.y - b2.y};
-			dst[3 * Ns] = cpx<R>{b1.x - b3.x, b1.y - b3.y};
-		}
-		__syncthreads();
-		cpx<R>* t = x;
-		x = y;
-		y = t;
-		ls += 2;
-	}
-	if (ls < log2n) {
-		const int Ns = 1 << ls;
-		const int half = n >> 1;
-		const int total = nb * half;
-		for (int q = threadIdx.x; q < total; q += blockDim.x) {
-			const int b = q >> (log2n - 1);
-			const int j = q & (half - 1);
-			const cpx<R> a0 = x[b * n + j];
-			const int k = j & (Ns - 1);
-			const cpx<R> a1 = twiddled(x[b * n + j + half], tw, k, sign);
-			const int d = ((j >> ls) << (ls + 1)) + k;
-			y[b * n + d] = cpx<R>{a0.x + a1.x, a0.y + a1.y};
-			y[b * n + d + Ns] = cpx<R>{a0.x - a1.x, a0.y - a1.y};
-		}
-		__syncthreads();
-		cpx<R>* t = x;
-		x = y;
-		y = t;
-	}
-	return x;
 }
 
 // -dzeta/dt at (it, x, y) (src/derivative.hh:19-70), evaluated in double.
@@ -158,57 +86,134 @@ __device__ __forceinline__ double neg_dzdt(const T* __restrict__ zeta, int nt, s
 	return -r;
 }
 
-// Rows of -zeta_t, transformed along y; the half row ky = 0..ny/2 is kept.
-template <class T>
-__global__ void fwd_rows_kernel(const T* __restrict__ zeta, int nt, int nx, int ny, int log2ny,
-                                const int* __restrict__ idx_t, double dt,
-                                const double2* __restrict__ twy, cpx<T>* __restrict__ spec,
-                                int rows) {
+// Exchange buffer index of the four-step transform: sequence s, output k2 of
+// the first DFTs, input n1 of the second.  Rows are padded so that both the
+// writes (n1 or s varying across lanes) and the reads (k2 varying) spread
+// over the LDS banks.
+template <int N1, int N2>
+__device__ __forceinline__ int xidx(int s, int k2, int n1) {
+	return s * (N2 * (N1 + 1) + 1) + k2 * (N1 + 1) + n1;
+}
+template <int N1, int N2>
+constexpr int xsize(int nseq) {
+	return nseq * (N2 * (N1 + 1) + 1);
+}
+
+// ---- forward spectrum -------------------------------------------------------
+
+// Rows of -zeta_t, real transform of length ny = 2M along y via a complex
+// transform of length M = N1*N2.  Writes the half spectrum k = 0..M
+// column-major: spec[i][k][x].  R rows per block, R*N1 <= 256.
+template <class T, int N1, int N2>
+__global__ __launch_bounds__(kThreads) void fwd_rows_kernel(
+	const T* __restrict__ zeta, int nt, int nx, int ny, const int* __restrict__ idx_t, double dt,
+	const double2* __restrict__ twy, cpx<T>* __restrict__ spec, int R) {
+	constexpr int M = N1 * N2;
 	extern __shared__ unsigned char smem_raw[];
-	cpx<T>* b0 = reinterpret_cast<cpx<T>*>(smem_raw);
-	cpx<T>* b1 = b0 + rows * ny;
+	cpx<T>* X = reinterpret_cast<cpx<T>*>(smem_raw);
 	const int i = blockIdx.y;
-	const int x0 = blockIdx.x * rows;
+	const int x0 = blockIdx.x * R;
 	const int it = idx_t[i];
 	const size_t slice = size_t(nx) * ny;
-	for (int q = threadIdx.x; q < rows * ny; q += blockDim.x) {
-		const int r = q / ny, y = q - r * ny;
+	const int tid = threadIdx.x;
+	if (tid < R * N1) {
+		const int n1 = tid % N1, r = tid / N1;
 		const int x = x0 + r;
-		T v = 0;
-		if (x < nx) v = T(neg_dzdt(zeta, nt, slice, it, size_t(x) * ny + y, dt));
-		b0[q] = cpx<T>{v, T(0)};
+		cpx<T> v[N2];
+#pragma unroll
+		for (int n2 = 0; n2 < N2; ++n2) {
+			const int n = n1 + N1 * n2;
+			v[n2] = cpx<T>{T(0), T(0)};
+			if (x < nx) {
+				const size_t off = size_t(x) * ny + 2 * n;
+				v[n2] = cpx<T>{T(neg_dzdt(zeta, nt, slice, it, off, dt)),
+				               T(neg_dzdt(zeta, nt, slice, it, off + 1, dt))};
+			}
+		}
+		dft<N2, -1>(v);
+#pragma unroll
+		for (int m = 0; m < N2; ++m) {
+			const int k2 = brev(m, N2);
+			X[xidx<N1, N2>(r, k2, n1)] = twiddle<-1>(v[m], twy, 2 * n1 * k2);
+		}
 	}
 	__syncthreads();
-	cpx<T>* res = fft_lds(b0, b1, ny, log2ny, rows, twy, -1);
-	const int nyh = ny / 2 + 1;
-	for (int q = threadIdx.x; q < rows * nyh; q += blockDim.x) {
-		const int r = q / nyh, ky = q - r * nyh;
+	// second DFTs (at most two tasks per lane: R*N2 <= 2*R*N1 <= 512)
+	cpx<T> w[2][N1];
+	int task[2];
+#pragma unroll
+	for (int q = 0; q < 2; ++q) {
+		task[q] = tid + q * kThreads;
+		if (task[q] < R * N2) {
+			const int k2 = task[q] % N2, r = task[q] / N2;
+#pragma unroll
+			for (int n1 = 0; n1 < N1; ++n1) w[q][n1] = X[xidx<N1, N2>(r, k2, n1)];
+		}
+	}
+	__syncthreads();
+	// Z[r][k] reuses the exchange buffer, row stride M + 1
+#pragma unroll
+	for (int q = 0; q < 2; ++q) {
+		if (task[q] < R * N2) {
+			const int k2 = task[q] % N2, r = task[q] / N2;
+			dft<N1, -1>(w[q]);
+#pragma unroll
+			for (int m = 0; m < N1; ++m) X[r * (M + 1) + k2 + N2 * brev(m, N1)] = w[q][m];
+		}
+	}
+	__syncthreads();
+	// H[k] = (Z[k] + conj Z[M-k])/2 - (i/2) e^{-i pi k/M} (Z[k] - conj Z[M-k])
+	for (int t = tid; t < R * (M + 1); t += kThreads) {
+		const int r = t % R, k = t / R;
 		const int x = x0 + r;
-		if (x < nx) spec[(size_t(i) * nx + x) * nyh + ky] = res[r * ny + ky];
+		if (x >= nx) continue;
+		const cpx<T> zk = X[r * (M + 1) + (k == M ? 0 : k)];
+		const cpx<T> zm = conj(X[r * (M + 1) + (k == 0 ? 0 : M - k)]);
+		const cpx<T> e = zk + zm;
+		const cpx<T> wd = twiddle<-1>(zk - zm, twy, k);
+		spec[(size_t(i) * (M + 1) + k) * nx + x] =
+			cpx<T>{T(0.5) * (e.x + wd.y), T(0.5) * (e.y - wd.x)};
 	}
 }
 
-// Columns of the half spectrum, transformed along x in place.
-template <class T>
-__global__ void fwd_cols_kernel(int nx, int log2nx, int nyh, const double2* __restrict__ twx,
-                                cpx<T>* __restrict__ spec, int cols) {
+// Columns k = 0..M of the half spectrum, forward transform of length
+// nx = N1*N2 along x, in place in spec[i][k][x].  S columns per block.
+template <class T, int N1, int N2>
+__global__ __launch_bounds__(kThreads) void fwd_cols_kernel(int nx, int M,
+                                                             const double2* __restrict__ twx,
+                                                             cpx<T>* __restrict__ spec, int S) {
 	extern __shared__ unsigned char smem_raw[];
-	cpx<T>* b0 = reinterpret_cast<cpx<T>*>(smem_raw);
-	cpx<T>* b1 = b0 + cols * nx;
+	cpx<T>* X = reinterpret_cast<cpx<T>*>(smem_raw);
 	const int i = blockIdx.y;
-	const int c0 = blockIdx.x * cols;
-	cpx<T>* base = spec + size_t(i) * nx * nyh;
-	for (int q = threadIdx.x; q < cols * nx; q += blockDim.x) {
-		const int kx = q / cols, c = q - kx * cols;
-		const int ky = c0 + c;
-		b0[c * nx + kx] = ky < nyh ? base[size_t(kx) * nyh + ky] : cpx<T>{T(0), T(0)};
+	const int c0 = blockIdx.x * S;
+	cpx<T>* base = spec + size_t(i) * (M + 1) * nx;
+	const int tid = threadIdx.x;
+	if (tid < S * N1) {
+		const int n1 = tid % N1, s = tid / N1;
+		const int col = c0 + s;
+		cpx<T> v[N2];
+#pragma unroll
+		for (int n2 = 0; n2 < N2; ++n2)
+			v[n2] = col <= M ? base[size_t(col) * nx + n1 + N1 * n2] : cpx<T>{T(0), T(0)};
+		dft<N2, -1>(v);
+#pragma unroll
+		for (int m = 0; m < N2; ++m) {
+			const int k2 = brev(m, N2);
+			X[xidx<N1, N2>(s, k2, n1)] = twiddle<-1>(v[m], twx, n1 * k2);
+		}
 	}
 	__syncthreads();
-	cpx<T>* res = fft_lds(b0, b1, nx, log2nx, cols, twx, -1);
-	for (int q = threadIdx.x; q < cols * nx; q += blockDim.x) {
-		const int kx = q / cols, c = q - kx * cols;
-		const int ky = c0 + c;
-		if (ky < nyh) base[size_t(kx) * nyh + ky] = res[c * nx + kx];
+	for (int t = tid; t < S * N2; t += kThreads) {
+		const int k2 = t % N2, s = t / N2;
+		const int col = c0 + s;
+		cpx<T> w[N1];
+#pragma unroll
+		for (int n1 = 0; n1 < N1; ++n1) w[n1] = X[xidx<N1, N2>(s, k2, n1)];
+		dft<N1, -1>(w);
+		if (col <= M) {
+#pragma unroll
+			for (int m = 0; m < N1; ++m) base[size_t(col) * nx + k2 + N2 * brev(m, N1)] = w[m];
+		}
 	}
 }
 
@@ -368,6 +373,8 @@ __global__ __launch_bounds__(256) void factor_waves_kernel(const T* __restrict__
 	}
 }
 
+// ---- window -----------------------------------------------------------------
+
 // Rotate map of blitz::rotate(W, shape/2): swaps j <-> j + n/2 for j < n/2.
 __device__ __forceinline__ int rot_index(int j, int n) {
 	const int m = n / 2;
@@ -376,151 +383,221 @@ __device__ __forceinline__ int rot_index(int j, int n) {
 	return j;
 }
 
+// W at one wave number along nz equally spaced levels z_j = zs + j dz:
+// W_j = scale (a_j + c_j), a_j = e^{l z_j}, c_j = e^{-l (z_j + 2h)}.
 struct WinParts {
-	double a, c, rho, irho, scale;  // W_j = scale * (a_j + c_j)
-	int jstart;                     // a_j = 0 for j < jstart (e^{l z} below 1e-304)
-	bool direct;                    // step too large: evaluate exponentials directly
-	double l;
+	double a, c, rho, irho, scale, l;
+	int jstart;   // a_j = 0 for j < jstart (e^{l z} below 1e-304)
+	bool direct;  // step too large: evaluate the exponentials directly
+
+	__device__ __forceinline__ void init(int px, int py, double du, double dv, double zs,
+	                                     double dz, double h, int nz) {
+		if (px == 0 && py == 0) {
+			px = 1;
+			py = 1;
+		}
+		const double u = du * px, v = dv * py;
+		l = kTwoPi * sqrt(u * u + v * v);
+		scale = kFourPi / (1.0 + exp(-2.0 * l * h));
+		direct = l * fabs(dz) > 600.0;
+		c = exp(-l * (zs + 2.0 * h));
+		rho = exp(l * dz);
+		irho = exp(-l * dz);
+		// first level where l z > -700
+		int js = 0;
+		if (l * zs < -700.0 && dz > 0.0) {
+			js = int(ceil((-700.0 / l - zs) / dz));
+			if (js < 0) js = 0;
+		}
+		jstart = js;
+		a = js < nz ? exp(l * (zs + dz * js)) : 0.0;
+	}
+	// W_j for j = 0, 1, 2, ... in order
+	__device__ __forceinline__ double next(int j, double zs, double dz, double h) {
+		double av, cv;
+		if (direct) {
+			const double zj = zs + dz * j;
+			av = exp(l * zj);
+			cv = exp(-l * (zj + 2.0 * h));
+		} else {
+			av = j < jstart ? 0.0 : a;
+			cv = c;
+			if (j >= jstart) a *= rho;
+			c *= irho;
+		}
+		return scale * (av + cv);
+	}
 };
 
-__device__ __forceinline__ WinParts win_parts(int px, int py, double du, double dv, double z0,
-                                              double dz, double h, int nz) {
-	if (px == 0 && py == 0) {
-		px = 1;
-		py = 1;
-	}
-	const double u = du * px, v = dv * py;
-	WinParts w;
-	w.l = kTwoPi * sqrt(u * u + v * v);
-	const double l = w.l;
-	w.scale = kFourPi / (1.0 + exp(-2.0 * l * h));
-	w.direct = l * fabs(dz) > 600.0;
-	w.c = exp(-l * (z0 + 2.0 * h));
-	w.rho = exp(l * dz);
-	w.irho = exp(-l * dz);
-	// first level where l z > -700
-	int js = 0;
-	if (l * z0 < -700.0 && dz > 0.0) {
-		js = int(ceil((-700.0 / l - z0) / dz));
-		if (js < 0) js = 0;
-	}
-	w.jstart = js;
-	w.a = js < nz ? exp(l * (z0 + dz * js)) : 0.0;
-	return w;
-}
+// ---- inverse transforms -------------------------------------------------------
 
-// Per element of a column block: multiply by W_sym(z) for every z and
-// transform along x (backward).  Output G[(i, z)][x][ky].
-template <class T, int EPT>
-__global__ __launch_bounds__(256) void inv_cols_kernel(
-	int nx, int log2nx, int ny, int nyh, int nz, int i0, const double2* __restrict__ twx,
+// Packed columns c0..c0+CB-1 of slice i (column 0 carries ky = 0 and ky = M),
+// times W_sym(z) for every level, inverse transform of length nx = N1*N2
+// along x.  Output G[(il, z)][c][x].  Levels go in groups of ZB through LDS;
+// ZB*CB*N1 <= 256.
+template <class T, int N1, int N2>
+__global__ __launch_bounds__(kThreads) void inv_cols_kernel(
+	int ny, int M, int nz, int i0, const double2* __restrict__ twx,
 	const cpx<T>* __restrict__ spec, const double* __restrict__ kmin, double z0, double dz,
-	double h, cpx<T>* __restrict__ G, int cols) {
+	double h, cpx<T>* __restrict__ G, int CB, int ZB) {
+	constexpr int N = N1 * N2;
+	constexpr int EPT = N > kThreads ? N / kThreads : 1;  // CB*N <= max(256, N)
 	extern __shared__ unsigned char smem_raw[];
-	cpx<T>* fin = reinterpret_cast<cpx<T>*>(smem_raw);
-	cpx<T>* b0 = fin + cols * nx;
-	cpx<T>* b1 = b0 + cols * nx;
+	cpx<T>* X = reinterpret_cast<cpx<T>*>(smem_raw);
 	const int il = blockIdx.y;
 	const int i = i0 + il;
-	const int c0 = blockIdx.x * cols;
-	const cpx<T>* base = spec + size_t(i) * nx * nyh;
-	for (int q = threadIdx.x; q < cols * nx; q += blockDim.x) {
-		const int kx = q / cols, c = q - kx * cols;
-		const int ky = c0 + c;
-		fin[c * nx + kx] = ky < nyh ? base[size_t(kx) * nyh + ky] : cpx<T>{T(0), T(0)};
+	const int c0 = blockIdx.x * CB;
+	const int tid = threadIdx.x;
+	const int nel = CB * N;
+	const cpx<T>* base = spec + size_t(i) * (M + 1) * N;
+	cpx<T> f0[EPT], f1[EPT];
+#pragma unroll
+	for (int e = 0; e < EPT; ++e) {
+		const int q = tid + e * kThreads;
+		f0[e] = f1[e] = cpx<T>{T(0), T(0)};
+		if (q < nel) {
+			const int c = q / N, kx = q % N;
+			f0[e] = base[size_t(c0 + c) * N + kx];
+			if (c0 + c == 0) f1[e] = base[size_t(M) * N + kx];
+		}
 	}
 	const double wx = 1.0 / kmin[4 * i + 0];
 	const double wy = 1.0 / kmin[4 * i + 1];
-	const double du = wx / double(nx - 1 > 1 ? nx - 1 : 1);
+	const double du = wx / double(N - 1 > 1 ? N - 1 : 1);
 	const double dv = wy / double(ny - 1 > 1 ? ny - 1 : 1);
-	WinParts wp[EPT][2];
-	const int nel = cols * nx;
-#pragma unroll
-	for (int e = 0; e < EPT; ++e) {
-		const int q = threadIdx.x + e * blockDim.x;
-		if (q < nel) {
-			const int c = q / nx, kx = q - c * nx;
-			const int ky = c0 + c;
-			const int kxm = kx == 0 ? 0 : nx - kx;
-			const int kym = ky == 0 ? 0 : ny - ky;
-			wp[e][0] = win_parts(rot_index(kx, nx), rot_index(ky, ny), du, dv, z0, dz, h, nz);
-			wp[e][1] = win_parts(rot_index(kxm, nx), rot_index(kym % ny, ny), du, dv, z0, dz, h, nz);
-		}
-	}
-	__syncthreads();
-	for (int j = 0; j < nz; ++j) {
+	for (int j0 = 0; j0 < nz; j0 += ZB) {
+		const int zb = min(ZB, nz - j0);
+		const double zs = z0 + dz * j0;
+		// F * W_sym for the levels of this group
 #pragma unroll
 		for (int e = 0; e < EPT; ++e) {
-			const int q = threadIdx.x + e * blockDim.x;
+			const int q = tid + e * kThreads;
 			if (q < nel) {
-				double wsum = 0.0;
-#pragma unroll
-				for (int s = 0; s < 2; ++s) {
-					WinParts& w = wp[e][s];
-					double av, cv;
-					if (w.direct) {
-						const double zj = z0 + dz * j;
-						av = exp(w.l * zj);
-						cv = exp(-w.l * (zj + 2.0 * h));
-					} else {
-						av = j < w.jstart ? 0.0 : w.a;
-						cv = w.c;
-						if (j >= w.jstart) w.a *= w.rho;
-						w.c *= w.irho;
+				const int c = q / N, kx = q % N;
+				const int ky = c0 + c;
+				const int kxm = kx == 0 ? 0 : N - kx;
+				cpx<T>* dst = X + c * N + kx;  // level zz at dst[zz * CB * N]
+				{
+					WinParts wa, wb;
+					wa.init(rot_index(kx, N), rot_index(ky, ny), du, dv, zs, dz, h, zb);
+					wb.init(rot_index(kxm, N), rot_index(ky == 0 ? 0 : ny - ky, ny), du, dv, zs,
+					        dz, h, zb);
+					for (int zz = 0; zz < zb; ++zz) {
+						const T w = T(0.5 * (wa.next(zz, zs, dz, h) + wb.next(zz, zs, dz, h)));
+						dst[zz * CB * N] = cpx<T>{f0[e].x * w, f0[e].y * w};
 					}
-					wsum += w.scale * (av + cv);
 				}
-				const T ws = T(0.5 * wsum);
-				const cpx<T> f = fin[q];
-				b0[q] = cpx<T>{f.x * ws, f.y * ws};
+				if (ky == 0) {
+					// packed column: add i * F(kx, M) W_sym(kx, M)
+					WinParts wa, wb;
+					wa.init(rot_index(kx, N), rot_index(M, ny), du, dv, zs, dz, h, zb);
+					wb.init(rot_index(kxm, N), rot_index(M, ny), du, dv, zs, dz, h, zb);
+					for (int zz = 0; zz < zb; ++zz) {
+						const T w = T(0.5 * (wa.next(zz, zs, dz, h) + wb.next(zz, zs, dz, h)));
+						const cpx<T> a = dst[zz * CB * N];
+						dst[zz * CB * N] = cpx<T>{a.x - f1[e].y * w, a.y + f1[e].x * w};
+					}
+				}
 			}
 		}
 		__syncthreads();
-		cpx<T>* res = fft_lds(b0, b1, nx, log2nx, cols, twx, +1);
-		cpx<T>* out = G + (size_t(il) * nz + j) * nx * nyh;
-		for (int q = threadIdx.x; q < nel; q += blockDim.x) {
-			const int kx = q / cols, c = q - kx * cols;
-			const int ky = c0 + c;
-			if (ky < nyh) out[size_t(kx) * nyh + ky] = res[c * nx + kx];
+		const int nseq = zb * CB;
+		const bool act = tid < nseq * N1;
+		const int n1 = tid % N1, s = tid / N1;
+		cpx<T> v[N2];
+		if (act) {
+#pragma unroll
+			for (int n2 = 0; n2 < N2; ++n2) v[n2] = X[s * N + n1 + N1 * n2];
+		}
+		__syncthreads();
+		if (act) {
+			dft<N2, +1>(v);
+#pragma unroll
+			for (int m = 0; m < N2; ++m) {
+				const int k2 = brev(m, N2);
+				X[xidx<N1, N2>(s, k2, n1)] = twiddle<+1>(v[m], twx, n1 * k2);
+			}
+		}
+		__syncthreads();
+		for (int t = tid; t < nseq * N2; t += kThreads) {
+			const int k2 = t % N2, sq = t / N2;
+			const int zz = sq / CB, c = sq % CB;
+			cpx<T> w[N1];
+#pragma unroll
+			for (int m = 0; m < N1; ++m) w[m] = X[xidx<N1, N2>(sq, k2, m)];
+			dft<N1, +1>(w);
+			cpx<T>* out = G + ((size_t(il) * nz + j0 + zz) * M + c0 + c) * N;
+#pragma unroll
+			for (int m = 0; m < N1; ++m) out[k2 + N2 * brev(m, N1)] = w[m];
 		}
 		__syncthreads();
 	}
 }
 
-// Rows: complex-to-real backward transform along y, scaled by 1/(nx ny).
-template <class T>
-__global__ void inv_rows_kernel(int nx, int ny, int log2ny, int nyh, int nz, int i0,
-                                const double2* __restrict__ twy, const cpx<T>* __restrict__ G,
-                                T* __restrict__ phi, int rows) {
+// Rows: real output of length ny = 2M from the M packed columns, through a
+// complex transform of length M = N1*N2, scaled by 1/(nx ny).  R rows per
+// block, R*N1 <= 256.
+template <class T, int N1, int N2>
+__global__ __launch_bounds__(kThreads) void inv_rows_kernel(int nx, int ny, int nz, int i0,
+                                                             const double2* __restrict__ twy,
+                                                             const cpx<T>* __restrict__ G,
+                                                             T* __restrict__ phi, int R) {
+	constexpr int M = N1 * N2;
 	extern __shared__ unsigned char smem_raw[];
-	cpx<T>* b0 = reinterpret_cast<cpx<T>*>(smem_raw);
-	cpx<T>* b1 = b0 + rows * ny;
-	const int item = blockIdx.y;  // (il, z) pair
-	const int x0 = blockIdx.x * rows;
-	const cpx<T>* g = G + size_t(item) * nx * nyh;
-	for (int q = threadIdx.x; q < rows * ny; q += blockDim.x) {
-		const int r = q / ny, y = q - r * ny;
+	cpx<T>* X = reinterpret_cast<cpx<T>*>(smem_raw);
+	const int item = blockIdx.y;  // (il, z)
+	const int x0 = blockIdx.x * R;
+	const cpx<T>* g = G + size_t(item) * M * nx;
+	const int tid = threadIdx.x;
+	if (tid < R * N1) {
+		const int r = tid % R, n1 = tid / R;
 		const int x = x0 + r;
-		cpx<T> v{T(0), T(0)};
-		if (x < nx) {
-			if (y < nyh) {
-				v = g[size_t(x) * nyh + y];
-			} else {
-				const cpx<T> u = g[size_t(x) * nyh + (ny - y)];
-				v = cpx<T>{u.x, -u.y};
+		cpx<T> v[N2];
+#pragma unroll
+		for (int n2 = 0; n2 < N2; ++n2) {
+			const int k = n1 + N1 * n2;
+			v[n2] = cpx<T>{T(0), T(0)};
+			if (x < nx) {
+				if (k == 0) {
+					// packed column: H[0] = real part, H[M] = imaginary part
+					const cpx<T> p = g[x];
+					v[n2] = cpx<T>{p.x + p.y, p.x - p.y};
+				} else {
+					// Z[k] = (H[k] + conj H[M-k]) + i e^{i pi k/M} (H[k] - conj H[M-k])
+					const cpx<T> hk = g[size_t(k) * nx + x];
+					const cpx<T> hm = conj(g[size_t(M - k) * nx + x]);
+					const cpx<T> sum = hk + hm;
+					const cpx<T> wd = twiddle<+1>(hk - hm, twy, k);
+					v[n2] = cpx<T>{sum.x - wd.y, sum.y + wd.x};
+				}
 			}
 		}
-		b0[q] = v;
+		dft<N2, +1>(v);
+#pragma unroll
+		for (int m = 0; m < N2; ++m) {
+			const int k2 = brev(m, N2);
+			X[xidx<N1, N2>(r, k2, n1)] = twiddle<+1>(v[m], twy, 2 * n1 * k2);
+		}
 	}
 	__syncthreads();
-	cpx<T>* res = fft_lds(b0, b1, ny, log2ny, rows, twy, +1);
 	const int il = item / nz, j = item - il * nz;
 	T* out = phi + ((size_t(i0 + il) * nz + j) * nx) * ny;
-	const double inv = 1.0 / (double(nx) * double(ny));
-	for (int q = threadIdx.x; q < rows * ny; q += blockDim.x) {
-		const int r = q / ny, y = q - r * ny;
+	const T inv = T(1.0 / (double(nx) * double(ny)));
+	for (int t = tid; t < R * N2; t += kThreads) {
+		const int k2 = t % N2, r = t / N2;
 		const int x = x0 + r;
-		if (x < nx) out[size_t(x) * ny + y] = T(double(res[q].x) * inv);
+		cpx<T> w[N1];
+#pragma unroll
+		for (int n1 = 0; n1 < N1; ++n1) w[n1] = X[xidx<N1, N2>(r, k2, n1)];
+		dft<N1, +1>(w);
+		if (x < nx) {
+#pragma unroll
+			for (int m = 0; m < N1; ++m) {
+				const int n = k2 + N2 * brev(m, N1);
+				*reinterpret_cast<cpx<T>*>(out + size_t(x) * ny + 2 * n) =
+					cpx<T>{w[m].x * inv, w[m].y * inv};
+			}
+		}
 	}
 }
 
@@ -557,111 +634,216 @@ __global__ void edge_kernel(int nx, int ny, T* __restrict__ phi, int i0, int nz)
 		f[l] = T(interp3(f, ny, 1, l - 1, 1, l, 2, l - 1, 0, l));
 }
 
-// Largest batch of length-n transforms whose LDS buffers fit in 64 KiB.
-int lds_batch(int n, size_t elem, int buffers, int cap) {
-	const size_t per = size_t(buffers) * elem * size_t(n);
-	int b = int(65536 / per);
-	return std::max(1, std::min(cap, b));
+// ---- launch plumbing ------------------------------------------------------------
+
+constexpr int split1(int n) {
+	int l = 0;
+	while ((1 << l) < n) ++l;
+	return 1 << (l / 2);
+}
+
+template <class K>
+void allow_lds(K kernel, size_t lds) {
+	if (lds > 65536)
+		ARMA_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+		                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+}
+
+struct PotPlan {
+	int nt, nx, ny, M, nto, nz;
+	double dt, z0, dz, h;
+};
+
+// Length-dependent launches, instantiated for every power of two.
+template <class T, int NX>
+struct ColLaunch {
+	static constexpr int N1 = split1(NX), N2 = NX / N1;
+
+	static void forward(arma_hip_ctx* ctx, const PotPlan& p, const double2* twx, cpx<T>* spec) {
+		const int S = std::min(p.M + 1, kThreads / N1);
+		const size_t lds = sizeof(cpx<T>) * size_t(xsize<N1, N2>(S));
+		allow_lds(&fwd_cols_kernel<T, N1, N2>, lds);
+		fwd_cols_kernel<T, N1, N2><<<dim3(div_ceil(p.M + 1, S), p.nto), kThreads, lds,
+		                             ctx->stream>>>(NX, p.M, twx, spec, S);
+		ARMA_HIP_CHECK(hipGetLastError());
+	}
+
+	static void inverse(arma_hip_ctx* ctx, const PotPlan& p, int i0, int cn, const double2* twx,
+	                    const cpx<T>* spec, const double* dk, cpx<T>* G) {
+		const int CB = std::max(1, std::min(p.M, kThreads / NX));
+		const size_t per_seq = sizeof(cpx<T>) * size_t(std::max(NX, xsize<N1, N2>(1)));
+		int ZB = std::min(p.nz, std::max(1, kThreads / N1 / CB));
+		while (ZB > 1 && size_t(ZB) * CB * per_seq > 81920) --ZB;
+		const size_t lds = size_t(ZB) * CB * per_seq;
+		allow_lds(&inv_cols_kernel<T, N1, N2>, lds);
+		inv_cols_kernel<T, N1, N2><<<dim3(div_ceil(p.M, CB), cn), kThreads, lds, ctx->stream>>>(
+			p.ny, p.M, p.nz, i0, twx, spec, dk, p.z0, p.dz, p.h, G, CB, ZB);
+		ARMA_HIP_CHECK(hipGetLastError());
+	}
+};
+
+template <class T, int M>
+struct RowLaunch {
+	static constexpr int N1 = split1(M), N2 = M / N1;
+
+	static int rows(const PotPlan& p) { return std::min(p.nx, kThreads / N1); }
+
+	static void forward(arma_hip_ctx* ctx, const PotPlan& p, const T* zeta, const int* didx,
+	                    const double2* twy, cpx<T>* spec) {
+		const int R = rows(p);
+		const size_t lds = sizeof(cpx<T>) * size_t(std::max(xsize<N1, N2>(R), R * (M + 1)));
+		allow_lds(&fwd_rows_kernel<T, N1, N2>, lds);
+		fwd_rows_kernel<T, N1, N2><<<dim3(div_ceil(p.nx, R), p.nto), kThreads, lds,
+		                             ctx->stream>>>(zeta, p.nt, p.nx, p.ny, didx, p.dt, twy,
+		                                            spec, R);
+		ARMA_HIP_CHECK(hipGetLastError());
+	}
+
+	static void inverse(arma_hip_ctx* ctx, const PotPlan& p, int i0, int cn, const double2* twy,
+	                    const cpx<T>* G, T* phi) {
+		const int R = rows(p);
+		const size_t lds = sizeof(cpx<T>) * size_t(xsize<N1, N2>(R));
+		allow_lds(&inv_rows_kernel<T, N1, N2>, lds);
+		inv_rows_kernel<T, N1, N2><<<dim3(div_ceil(p.nx, R), cn * p.nz), kThreads, lds,
+		                             ctx->stream>>>(p.nx, p.ny, p.nz, i0, twy, G, phi, R);
+		ARMA_HIP_CHECK(hipGetLastError());
+	}
+};
+
+// Calls F<L>::run(args...) for the power of two L = n in [lo, 1024].
+template <template <int> class F, int L = 2, class... A>
+void by_length(int n, A&&... args) {
+	if constexpr (L <= 1024) {
+		if (n == L) {
+			F<L>::run(std::forward<A>(args)...);
+			return;
+		}
+		by_length<F, L * 2>(n, std::forward<A>(args)...);
+	} else {
+		ARMA_HIP_REQUIRE(false, ARMA_HIP_EINVAL, "transform length must be a power of two");
+	}
+}
+
+template <class T>
+struct Ops {
+	template <int L>
+	struct ColFwd {
+		static void run(arma_hip_ctx* c, const PotPlan& p, const double2* tw, cpx<T>* s) {
+			if constexpr (L >= 4) ColLaunch<T, L>::forward(c, p, tw, s);
+		}
+	};
+	template <int L>
+	struct ColInv {
+		static void run(arma_hip_ctx* c, const PotPlan& p, int i0, int cn, const double2* tw,
+		                const cpx<T>* s, const double* dk, cpx<T>* G) {
+			if constexpr (L >= 4) ColLaunch<T, L>::inverse(c, p, i0, cn, tw, s, dk, G);
+		}
+	};
+	template <int L>
+	struct RowFwd {
+		static void run(arma_hip_ctx* c, const PotPlan& p, const T* z, const int* di,
+		                const double2* tw, cpx<T>* s) {
+			if constexpr (L <= 512) RowLaunch<T, L>::forward(c, p, z, di, tw, s);
+		}
+	};
+	template <int L>
+	struct RowInv {
+		static void run(arma_hip_ctx* c, const PotPlan& p, int i0, int cn, const double2* tw,
+		                const cpx<T>* G, T* phi) {
+			if constexpr (L <= 512) RowLaunch<T, L>::inverse(c, p, i0, cn, tw, G, phi);
+		}
+	};
+};
+
+// Bytes of the inverse intermediate per pass: small enough to stay in the
+// last-level cache between the two passes.  ARMA_HIP_POT_CHUNK_MB overrides.
+size_t chunk_bytes() {
+	if (const char* e = std::getenv("ARMA_HIP_POT_CHUNK_MB")) {
+		const long v = std::atol(e);
+		if (v > 0) return size_t(v) << 20;
+	}
+	return size_t(128) << 20;
 }
 
 template <class T>
 void run_potential(arma_hip_ctx* ctx, const arma_pot_desc* d, const T* zeta, T* phi,
                    double* wnmax_out) {
-	const int nt = d->shape[0], nx = d->shape[1], ny = d->shape[2];
-	const int lx = ilog2_exact(nx), ly = ilog2_exact(ny);
-	ARMA_HIP_REQUIRE(lx >= 2 && ly >= 2 && nx <= 1024 && ny <= 1024, ARMA_HIP_EINVAL,
+	PotPlan p;
+	p.nt = d->shape[0];
+	p.nx = d->shape[1];
+	p.ny = d->shape[2];
+	const int lx = ilog2_exact(p.nx), ly = ilog2_exact(p.ny);
+	ARMA_HIP_REQUIRE(lx >= 2 && ly >= 2 && p.nx <= 1024 && p.ny <= 1024, ARMA_HIP_EINVAL,
 	                 "surface extents along x and y must be powers of two in [4, 1024]");
-	const int nto = d->npoints[0], nz = d->npoints[1];
-	const int nyh = ny / 2 + 1;
-	const double dtd = (d->ubound[0] - d->lbound[0]) / double(std::max(nto - 1, 1));
-	const double dz = (d->ubound[1] - d->lbound[1]) / double(std::max(nz - 1, 1));
-	std::vector<int> idx(nto), tint(nto);
-	for (int i = 0; i < nto; ++i) {
+	p.M = p.ny / 2;
+	p.nto = d->npoints[0];
+	p.nz = d->npoints[1];
+	p.dt = d->delta[0];
+	p.z0 = d->lbound[1];
+	p.h = d->depth;
+	const double dtd = (d->ubound[0] - d->lbound[0]) / double(std::max(p.nto - 1, 1));
+	p.dz = (d->ubound[1] - d->lbound[1]) / double(std::max(p.nz - 1, 1));
+	std::vector<int> idx(p.nto), tint(p.nto);
+	for (int i = 0; i < p.nto; ++i) {
 		const double tv = d->lbound[0] + dtd * i;
 		idx[i] = int(tv);
-		ARMA_HIP_REQUIRE(idx[i] >= 0 && idx[i] < nt, ARMA_HIP_EINVAL,
+		ARMA_HIP_REQUIRE(idx[i] >= 0 && idx[i] < p.nt, ARMA_HIP_EINVAL,
 		                 "potential domain time index outside the surface");
 		tint[i] = int(d->delta[0] * idx[i]);
 	}
 	const hipStream_t st = ctx->stream;
 	// twiddles (FP64, generated on the device)
-	double2* twx = (double2*)ctx->pot_twiddle.get(sizeof(double2) * size_t(nx + ny));
-	double2* twy = twx + nx;
-	twiddle_kernel<<<div_ceil(nx, 256), 256, 0, st>>>(twx, nx);
-	twiddle_kernel<<<div_ceil(ny, 256), 256, 0, st>>>(twy, ny);
-	int* didx = (int*)ctx->pot_wn.get(sizeof(int) * 2 * size_t(nto));
-	int* dtint = didx + nto;
+	double2* twx = (double2*)ctx->pot_twiddle.get(sizeof(double2) * size_t(p.nx + p.ny));
+	double2* twy = twx + p.nx;
+	twiddle_kernel<<<div_ceil(p.nx, 256), 256, 0, st>>>(twx, p.nx);
+	twiddle_kernel<<<div_ceil(p.ny, 256), 256, 0, st>>>(twy, p.ny);
+	int* didx = (int*)ctx->pot_wn.get(sizeof(int) * 2 * size_t(p.nto));
+	int* dtint = didx + p.nto;
 	std::vector<int> both(idx);
 	both.insert(both.end(), tint.begin(), tint.end());
-	upload(ctx, didx, both.data(), sizeof(int) * 2 * size_t(nto));
-	double* dk = (double*)ctx->pot_kmin.get(sizeof(double) * 4 * size_t(nto));
-	KernelTimer t_fw(ctx, "pot_factor_waves");
-	factor_waves_kernel<T><<<dim3(nto, 2), 256, 0, st>>>(zeta, nx, ny, didx, dtint, d->delta[0],
-	                                                     dk);
-	ARMA_HIP_CHECK(hipGetLastError());
-	// forward half spectra of every output slice
-	cpx<T>* spec = (cpx<T>*)ctx->pot_spec.get(sizeof(cpx<T>) * size_t(nto) * nx * nyh);
+	upload(ctx, didx, both.data(), sizeof(int) * 2 * size_t(p.nto));
+	double* dk = (double*)ctx->pot_kmin.get(sizeof(double) * 4 * size_t(p.nto));
 	{
-		const int rows = lds_batch(ny, sizeof(cpx<T>), 2, 8);
-		const size_t lds = 2 * sizeof(cpx<T>) * size_t(rows) * ny;
-		KernelTimer t_fwd(ctx, "pot_forward");
-		fwd_rows_kernel<T><<<dim3(div_ceil(nx, rows), nto), 256, lds, st>>>(
-			zeta, nt, nx, ny, ly, didx, d->delta[0], twy, spec, rows);
-		ARMA_HIP_CHECK(hipGetLastError());
-		const int cols = lds_batch(nx, sizeof(cpx<T>), 2, 8);
-		const size_t lds2 = 2 * sizeof(cpx<T>) * size_t(cols) * nx;
-		fwd_cols_kernel<T><<<dim3(div_ceil(nyh, cols), nto), 256, lds2, st>>>(nx, lx, nyh, twx,
-		                                                                       spec, cols);
+		KernelTimer t_fw(ctx, "pot_factor_waves");
+		factor_waves_kernel<T><<<dim3(p.nto, 2), 256, 0, st>>>(zeta, p.nx, p.ny, didx, dtint,
+		                                                       d->delta[0], dk);
 		ARMA_HIP_CHECK(hipGetLastError());
 	}
+	// forward half spectra of every output slice: spec[i][k][kx], k = 0..M
+	cpx<T>* spec = (cpx<T>*)ctx->pot_spec.get(sizeof(cpx<T>) * size_t(p.nto) * (p.M + 1) * p.nx);
+	{
+		KernelTimer t_fwd(ctx, "pot_forward");
+		by_length<Ops<T>::template RowFwd>(p.M, ctx, p, zeta, (const int*)didx,
+		                                   (const double2*)twy, spec);
+		by_length<Ops<T>::template ColFwd>(p.nx, ctx, p, (const double2*)twx, spec);
+	}
 	// inverse transforms in chunks of output slices
-	const size_t per_slice = size_t(nz) * nx * nyh * sizeof(cpx<T>);
-	int chunk = int(std::max<size_t>(1, (size_t(1) << 30) / per_slice));
-	chunk = std::min(chunk, nto);
+	const size_t per_slice = size_t(p.nz) * p.M * p.nx * sizeof(cpx<T>);
+	int chunk = int(std::max<size_t>(1, chunk_bytes() / per_slice));
+	chunk = std::min(chunk, p.nto);
 	cpx<T>* G = (cpx<T>*)ctx->pot_mid.get(per_slice * chunk);
-	const int cols = lds_batch(nx, sizeof(cpx<T>), 3, std::max(1, 1024 / nx));
-	const int ept = div_ceil(cols * nx, 256);
-	ARMA_HIP_REQUIRE(ept <= 4, ARMA_HIP_ERUNTIME, "column batch too large");
-	const size_t lds_c = 3 * sizeof(cpx<T>) * size_t(cols) * nx;
-	const int rows = lds_batch(ny, sizeof(cpx<T>), 2, 8);
-	const size_t lds_r = 2 * sizeof(cpx<T>) * size_t(rows) * ny;
-	for (int i0 = 0; i0 < nto; i0 += chunk) {
-		const int cn = std::min(chunk, nto - i0);
-		const dim3 gc(div_ceil(nyh, cols), cn);
+	for (int i0 = 0; i0 < p.nto; i0 += chunk) {
+		const int cn = std::min(chunk, p.nto - i0);
 		{
-		KernelTimer t_ic(ctx, "pot_inv_cols");
-		switch (ept) {
-		case 1:
-			inv_cols_kernel<T, 1><<<gc, 256, lds_c, st>>>(nx, lx, ny, nyh, nz, i0, twx, spec, dk,
-			                                             d->lbound[1], dz, d->depth, G, cols);
-			break;
-		case 2:
-			inv_cols_kernel<T, 2><<<gc, 256, lds_c, st>>>(nx, lx, ny, nyh, nz, i0, twx, spec, dk,
-			                                             d->lbound[1], dz, d->depth, G, cols);
-			break;
-		default:
-			inv_cols_kernel<T, 4><<<gc, 256, lds_c, st>>>(nx, lx, ny, nyh, nz, i0, twx, spec, dk,
-			                                             d->lbound[1], dz, d->depth, G, cols);
-			break;
-		}
-		ARMA_HIP_CHECK(hipGetLastError());
+			KernelTimer t_ic(ctx, "pot_inv_cols");
+			by_length<Ops<T>::template ColInv>(p.nx, ctx, p, i0, cn, (const double2*)twx,
+			                                   (const cpx<T>*)spec, (const double*)dk, G);
 		}
 		{
-		KernelTimer t_ir(ctx, "pot_inv_rows");
-		inv_rows_kernel<T><<<dim3(div_ceil(nx, rows), cn * nz), 256, lds_r, st>>>(
-			nx, ny, ly, nyh, nz, i0, twy, G, phi, rows);
-		ARMA_HIP_CHECK(hipGetLastError());
+			KernelTimer t_ir(ctx, "pot_inv_rows");
+			by_length<Ops<T>::template RowInv>(p.M, ctx, p, i0, cn, (const double2*)twy,
+			                                   (const cpx<T>*)G, phi);
 		}
 		KernelTimer t_ed(ctx, "pot_edge");
-		edge_kernel<T><<<cn * nz, 256, 0, st>>>(nx, ny, phi, i0, nz);
+		edge_kernel<T><<<cn * p.nz, 256, 0, st>>>(p.nx, p.ny, phi, i0, p.nz);
 		ARMA_HIP_CHECK(hipGetLastError());
 	}
 	if (wnmax_out) {
-		std::vector<double> hk(4 * size_t(nto));
-		ARMA_HIP_CHECK(hipMemcpyAsync(hk.data(), dk, sizeof(double) * 4 * nto,
+		std::vector<double> hk(4 * size_t(p.nto));
+		ARMA_HIP_CHECK(hipMemcpyAsync(hk.data(), dk, sizeof(double) * 4 * p.nto,
 		                              hipMemcpyDeviceToHost, st));
 		ARMA_HIP_CHECK(hipStreamSynchronize(st));
-		for (int i = 0; i < nto; ++i) {
+		for (int i = 0; i < p.nto; ++i) {
 			wnmax_out[2 * i] = 1.0 / hk[4 * i];
 			wnmax_out[2 * i + 1] = 1.0 / hk[4 * i + 1];
 		}
