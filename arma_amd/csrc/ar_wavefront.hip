@@ -16,12 +16,19 @@
 //  * LDS holds a ring of R = P1+P2-1 slices of the tile plus its causal halo
 //    (P1-1 rows above, P2-1 columns to the left): slot t mod R of position p
 //    is rewritten only after every reader of slice t is done.
-//  * Halo values come from the neighbouring tiles through HBM.  A tile
-//    publishes the number of steps it has completed (agent-scope release
-//    after all its stores); a consumer acquires its neighbours' progress
-//    before issuing the halo loads, which are prefetched D steps ahead.
-//    Tiles only wait on tiles with smaller workgroup ids, so the launch
-//    cannot deadlock.
+//  * Halo values come from the neighbouring tiles through HBM.  Surface
+//    stores and halo loads are device-coherent (agent-scope relaxed atomics:
+//    they write through, and read past, the per-XCD L2), so publishing needs
+//    no cache-wide write-back or invalidate: every wave waits for its stores
+//    to complete, the workgroup meets at a barrier, and one lane stores the
+//    number of steps completed.  A consumer polls its neighbours' counters
+//    (relaxed loads, with an acquire load every 64 polls) before issuing the
+//    halo loads, which are prefetched D steps ahead.  Tiles only wait on
+//    tiles with smaller workgroup ids, so the launch cannot deadlock.
+//  * In-slice taps go in chunks of 16: the chunk's ring values are read from
+//    LDS into registers once, and the coefficients (uniform across the
+//    workgroup) arrive through the scalar cache in rows of 16 and enter the
+//    FMAs as scalar operands.  Per-lane LDS traffic is one value per tap.
 //  * FP32 mode keeps every accumulator as a compensated (Kahan/TwoSum) pair;
 //    each step adds one P1*P2-term FP32 partial to it.
 #include "common.hpp"
@@ -35,8 +42,7 @@ namespace {
 constexpr int kTX = 16;
 constexpr int kTY = 16;
 constexpr int kThreads = kTX * kTY;
-constexpr int kD = 4;        // prefetch distance in steps
-constexpr int kB = 8;        // steps between progress publications / polls
+constexpr int kD = 4;  // prefetch distance of the noise and halo loads, in steps
 constexpr int kMaxP12 = 10;  // largest P1, P2 handled
 
 template <class T, bool KAHAN>
@@ -72,16 +78,51 @@ __global__ void reset_progress(int* p, int n) {
 	if (i < n) p[i] = 0;
 }
 
-__device__ __forceinline__ int load_progress(const int* p) {
-	return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+// Spin until *p >= need and return the value seen: relaxed device-coherent
+// polls, with an acquire load every 64 polls.
+__device__ __forceinline__ int wait_progress(const int* p, int need) {
+	for (int n = 1;; ++n) {
+		const int v = (n & 63) ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+		                       : __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+		if (v >= need) return v;
+		__builtin_amdgcn_s_sleep(1);
+	}
 }
+
+template <class T>
+__device__ __forceinline__ T coherent_load(const T* p) {
+	return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <class T>
+__device__ __forceinline__ void coherent_store(T* p, T v) {
+	__hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int pmod(int a, int r) {
+	const int m = a % r;
+	return m < 0 ? m + r : m;
+}
+
+__device__ __forceinline__ int wrap_inc(int v, int r) { return v + 1 == r ? 0 : v + 1; }
+
+// In-slice taps per chunk of the tap loop.  A chunk's kCH ring values are
+// read from LDS into registers once; its coefficients come in k-rows of kCH
+// through the scalar cache and enter the FMAs as scalar operands, so the
+// per-lane LDS traffic is one value per tap.
+constexpr int kCH = 16;
+
+__host__ __device__ constexpr int tap_chunks(int p12) { return (p12 - 1 + kCH - 1) / kCH; }
 
 template <class T, int P0, bool KAHAN>
 __global__ __launch_bounds__(kThreads) void ar_wave_kernel(
-	T* __restrict__ zeta, int nt, int nx, int ny, const T* __restrict__ phi, int P1, int P2,
-	int nY, int* __restrict__ progress, int total_steps) {
-	extern __shared__ unsigned char smem_raw[];
-	T* ring = reinterpret_cast<T*>(smem_raw);
+	T* __restrict__ zeta, int nt, int nx, int ny, const T* __restrict__ phi0,
+	const T* __restrict__ phic, int P1, int P2, int nY, int* __restrict__ progress,
+	const T* __restrict__ zero, T* __restrict__ sink, int total_steps) {
+	extern __shared__ __align__(16) unsigned char smem_raw[];
+	const int P12 = P1 * P2;
+	T* lzero = reinterpret_cast<T*>(smem_raw);  // 16 zero bytes
+	T* ring = lzero + 16 / sizeof(T);
 	const int HX = P1 - 1, HY = P2 - 1;
 	const int WX = kTX + HX, WY = kTY + HY;
 	const int R = P1 + P2 - 1;
@@ -127,132 +168,163 @@ __global__ __launch_bounds__(kThreads) void ar_wave_kernel(
 			if (hin[r]) hoff[r] = size_t(gx) * ny + gy;
 		}
 	}
-	// neighbour tiles and how far ahead they must be (in their steps)
+	// Neighbour tiles.  The corner tile needs no poll: the top tile had
+	// observed it far enough ahead before publishing the progress we wait for.
 	const int nb_top = X > 0 ? b - nY : -1;
 	const int nb_left = Y > 0 ? b - 1 : -1;
-	const int nb_corner = (X > 0 && Y > 0) ? b - nY - 1 : -1;
 
 	Acc<T, KAHAN> acc[P0];
 #pragma unroll
 	for (int k = 0; k < P0; ++k) acc[k].zero();
-	const T phi000 = phi[0];
+	const T phi000 = phi0[0];
 
 	// Step u of the loop is skew step s = u + s0; s0 < 0 lets the halo
 	// positions (negative skew hx + hy) deliver their first slices.  Progress
-	// counts completed u-steps; every tile uses the same s0.
+	// counts u-steps whose surface stores are complete; every tile uses the
+	// same s0.
 	const int s0 = -(HX + HY);
 	const int total_u = total_steps - s0;
+	// halo loads consumed at u-step u read points the neighbours computed at
+	// their u-steps up to u + kTX (top) and u + kTY (left)
+	auto need_top = [&](int u) { return min(u + kTX + 1, total_u); };
+	auto need_left = [&](int u) { return min(u + kTY + 1, total_u); };
 
 	// the ring starts zeroed: halo positions outside the grid stay zero
 	for (int q = tid; q < R * slot; q += kThreads) ring[q] = T(0);
+	if (tid < 16 / int(sizeof(T))) lzero[tid] = T(0);
+	// ring slots (slice t lives in slot t mod R), advanced every step
+	int rs_own = pmod(s0 - lx - ly, R);
+	int rs_h0 = pmod(s0 - hsk[0], R);
+	int rs_h1 = pmod(s0 - hsk[1], R);
 
-	// prefetch queues: noise of the own point and halo values, kD steps deep
+	// Every lane issues the same global loads and stores at every step
+	// (out-of-range loads read a zero, out-of-range stores go to this tile's
+	// scratch sink), so the vector memory counter waits for exactly the queue
+	// slot being consumed and no load result needs a select at issue.
+	T* const my_sink = sink + size_t(b) * kThreads + tid;
+	auto issue_loads = [&](int u2, T& e, T& h0, T& h1) {
+		const int s2 = u2 + s0;
+		const bool in = u2 < total_u;
+		const int t = s2 - lx - ly;
+		const bool ve = in && active && t >= 0 && t < nt;
+		const int t0 = s2 - hsk[0];
+		const bool v0 = in && hin[0] && t0 >= 0 && t0 < nt;
+		const int t1 = s2 - hsk[1];
+		const bool v1 = in && hin[1] && t1 >= 0 && t1 < nt;
+		e = *(ve ? zeta + size_t(t) * sxy + own_off : zero);
+		h0 = coherent_load(v0 ? zeta + size_t(t0) * sxy + hoff[0] : zero);
+		h1 = coherent_load(v1 ? zeta + size_t(t1) * sxy + hoff[1] : zero);
+	};
+
+	// neighbour progress as last observed by lane 0, and the flag loads in
+	// flight (read kD steps after they are issued)
+	int known_t = total_u, known_l = total_u;
+	if (tid == 0) {
+		known_t = nb_top >= 0 ? wait_progress(progress + nb_top, need_top(kD - 1)) : total_u;
+		known_l = nb_left >= 0 ? wait_progress(progress + nb_left, need_left(kD - 1)) : total_u;
+	}
+	__syncthreads();
 	T qe[kD], qh0[kD], qh1[kD];
+	int qt[kD], ql[kD];
 #pragma unroll
 	for (int d = 0; d < kD; ++d) {
-		qe[d] = T(0);
-		qh0[d] = T(0);
-		qh1[d] = T(0);
+		issue_loads(d, qe[d], qh0[d], qh1[d]);
+		qt[d] = known_t;
+		ql[d] = known_l;
 	}
-	int ready_to = 0;  // loads for u-steps < ready_to are covered by acquired progress
 
-	auto wait_neighbours = [&](int last_u) {
-		// make the halo values consumed at u-steps <= last_u readable
-		if (tid == 0) {
-			const int need_t = min(last_u + 1 + kTX, total_u);
-			const int need_l = min(last_u + 1 + kTY, total_u);
-			const int need_c = min(last_u + 1 + kTX + kTY, total_u);
-			if (nb_top >= 0)
-				while (load_progress(progress + nb_top) < need_t) __builtin_amdgcn_s_sleep(1);
-			if (nb_left >= 0)
-				while (load_progress(progress + nb_left) < need_l) __builtin_amdgcn_s_sleep(1);
-			if (nb_corner >= 0)
-				while (load_progress(progress + nb_corner) < need_c) __builtin_amdgcn_s_sleep(1);
-		}
-	};
-
-	auto issue_loads = [&](int u2, T& e, T& h0, T& h1) {
-		// loads that u-step u2 will consume
-		const int s2 = u2 + s0;
-		const int t = s2 - lx - ly;
-		e = (active && t >= 0 && t < nt) ? zeta[size_t(t) * sxy + own_off] : T(0);
-		const int t0 = s2 - hsk[0];
-		h0 = (hval[0] && hin[0] && t0 >= 0 && t0 < nt) ? zeta[size_t(t0) * sxy + hoff[0]] : T(0);
-		const int t1 = s2 - hsk[1];
-		h1 = (hval[1] && hin[1] && t1 >= 0 && t1 < nt) ? zeta[size_t(t1) * sxy + hoff[1]] : T(0);
-	};
-
-	// prologue: loads for u-steps 0..kD-1
-	wait_neighbours(kD - 1 + kB - 1);
-	ready_to = kD + kB - 1;
-	__syncthreads();
-#pragma unroll
-	for (int d = 0; d < kD; ++d) issue_loads(d, qe[d], qh0[d], qh1[d]);
-
-	auto step = [&](int u, T& e_q, T& h0_q, T& h1_q) {
+	auto step = [&](int u, T& e_q, T& h0_q, T& h1_q, int& t_q, int& l_q) {
 		const int s = u + s0;
 		// 1. halo values of step s into the ring
-		if (hval[0]) {
+		{
 			const int t0 = s - hsk[0];
-			if (t0 >= 0 && t0 < nt) ring[(t0 % R) * slot + hlds[0]] = h0_q;
-		}
-		if (hval[1]) {
+			if (hval[0] && t0 >= 0 && t0 < nt) ring[rs_h0 * slot + hlds[0]] = h0_q;
 			const int t1 = s - hsk[1];
-			if (t1 >= 0 && t1 < nt) ring[(t1 % R) * slot + hlds[1]] = h1_q;
+			if (hval[1] && t1 >= 0 && t1 < nt) ring[rs_h1 * slot + hlds[1]] = h1_q;
 		}
-		// 2. own point
+		// 2. own point, slice t = s - lx - ly (computed on every lane; only
+		//    live lanes write the ring and the surface)
 		const int t = s - lx - ly;
-		if (active && t >= 0 && t < nt) {
-			const T* base = ring + (t % R) * slot + own_lds;
+		const bool live = active && t >= 0 && t < nt;
+		{
+			const T* base = ring + rs_own * slot + own_lds;
 			T part[P0];
 #pragma unroll
 			for (int k = 0; k < P0; ++k) part[k] = T(0);
-			for (int i = 0; i < P1; ++i) {
-				for (int j = 0; j < P2; ++j) {
-					if (i == 0 && j == 0) continue;
-					const T v = base[-(i * WY + j)];
+			// taps m = 1 .. P12-1 in chunks of kCH; taps past the last one
+			// have zero coefficients and read a zero
+			int ci = 0, cj = 0;  // (i, j) of the last tap read
+			const T* pc = phic;
+			for (int c0 = 1; c0 < P12; c0 += kCH, pc += P0 * kCH) {
+				T v[kCH];
 #pragma unroll
-					for (int k = 0; k < P0; ++k) part[k] = fma(phi[(k * P1 + i) * P2 + j], v, part[k]);
+				for (int q = 0; q < kCH; ++q) {
+					if (++cj == P2) {
+						cj = 0;
+						++ci;
+					}
+					v[q] = *(c0 + q < P12 ? base - (ci * WY + cj) : lzero);
+				}
+#pragma unroll
+				for (int k = 0; k < P0; ++k) {
+#pragma unroll
+					for (int q = 0; q < kCH; ++q) part[k] = fma(pc[k * kCH + q], v[q], part[k]);
 				}
 			}
 			acc[0].add(part[0]);
 			const T eps = e_q;
 			const T z = eps + (acc[0].value() + phi000 * eps);
-			ring[(t % R) * slot + own_lds] = z;
-			zeta[size_t(t) * sxy + own_off] = z;
+			if (live) ring[rs_own * slot + own_lds] = z;
+			coherent_store(live ? zeta + size_t(t) * sxy + own_off : my_sink, z);
 #pragma unroll
-			for (int k = 1; k < P0; ++k) acc[k].add(fma(phi[k * P1 * P2], z, part[k]));
+			for (int k = 1; k < P0; ++k) acc[k].add(fma(phi0[k], z, part[k]));
 #pragma unroll
 			for (int k = 0; k + 1 < P0; ++k) acc[k] = acc[k + 1];
 			acc[P0 - 1].zero();
 		}
-		// 3. acquire neighbour progress for the next kB steps of loads
-		const int u_load = u + kD;
-		if (u_load >= ready_to && u_load < total_u) {
-			wait_neighbours(u_load + kB - 1);
-			ready_to = u_load + kB;
+		rs_own = wrap_inc(rs_own, R);
+		rs_h0 = wrap_inc(rs_h0, R);
+		rs_h1 = wrap_inc(rs_h1, R);
+		// 3. lane 0: may the loads of step u + kD go out?
+		const int u2 = u + kD;
+		if (tid == 0) {
+			known_t = max(known_t, t_q);
+			known_l = max(known_l, l_q);
+			if (u2 < total_u) {
+				if (known_t < need_top(u2))
+					known_t = wait_progress(progress + nb_top, need_top(u2));
+				if (known_l < need_left(u2))
+					known_l = wait_progress(progress + nb_left, need_left(u2));
+			}
+			t_q = nb_top >= 0 ? coherent_load(progress + nb_top) : total_u;
+			l_q = nb_left >= 0 ? coherent_load(progress + nb_left) : total_u;
 		}
-		// 4. publish own progress
-		const bool publish = ((u + 1) % kB == 0) || (u + 1 == total_u);
-		if (publish) __threadfence();
 		__syncthreads();
-		if (publish && tid == 0)
-			__hip_atomic_store(progress + b, u + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+		// 4. publish: the stores of step u - kD were issued before the queue
+		//    loads consumed at this step, so every wave has seen them complete
+		if (tid == 0 && u >= kD) coherent_store(progress + b, u - kD + 1);
 		// 5. loads for step u + kD into the queue slot just consumed
-		if (u_load < total_u) issue_loads(u_load, e_q, h0_q, h1_q);
+		issue_loads(u2, e_q, h0_q, h1_q);
 	};
 
 	int u = 0;
 	for (; u + kD <= total_u; u += kD) {
 #pragma unroll
-		for (int d = 0; d < kD; ++d) step(u + d, qe[d], qh0[d], qh1[d]);
+		for (int d = 0; d < kD; ++d) step(u + d, qe[d], qh0[d], qh1[d], qt[d], ql[d]);
 	}
+	static_assert(kD == 4, "the tail below names every queue slot");
 	for (int d = 0; u < total_u; ++u, ++d) {
-		if (d == 0) step(u, qe[0], qh0[0], qh1[0]);
-		else if (d == 1) step(u, qe[1], qh0[1], qh1[1]);
-		else if (d == 2) step(u, qe[2], qh0[2], qh1[2]);
-		else step(u, qe[3], qh0[3], qh1[3]);
+		switch (d) {
+#define ARMA_TAIL(D) \
+	case D: step(u, qe[D], qh0[D], qh1[D], qt[D], ql[D]); break;
+			ARMA_TAIL(0) ARMA_TAIL(1) ARMA_TAIL(2) ARMA_TAIL(3)
+#undef ARMA_TAIL
+		}
 	}
+	// every store complete, then the final count
+	__builtin_amdgcn_s_waitcnt(0);
+	__syncthreads();
+	if (tid == 0) coherent_store(progress + b, total_u);
 }
 
 template <class T, bool KAHAN>
@@ -261,18 +333,39 @@ void launch_typed(arma_hip_ctx* ctx, const arma_ar_desc* d, T* zeta) {
 	const int P0 = d->phi_shape[0], P1 = d->phi_shape[1], P2 = d->phi_shape[2];
 	ARMA_HIP_REQUIRE(P1 <= kMaxP12 && P2 <= kMaxP12 && P0 <= 10, ARMA_HIP_EINVAL,
 	                 "AR order above 10 is not supported by the wavefront kernel");
-	const size_t nphi = size_t(P0) * P1 * P2;
-	std::vector<T> ph(d->phi, d->phi + nphi);
-	T* dphi = (T*)ctx->phi.get(nphi * sizeof(T));
-	upload(ctx, dphi, ph.data(), nphi * sizeof(T));
+	// phi0[k] = phi(k,0,0); phic[ch][k][q] = phi(k, m) for the in-slice tap
+	// m = i*P2 + j = 1 + ch*kCH + q (zero past the last tap)
+	const int P12 = P1 * P2;
+	const int nch = tap_chunks(P12);
+	const size_t n0 = (size_t(P0) + 15) / 16 * 16;
+	std::vector<T> ph(n0 + size_t(nch) * P0 * kCH, T(0));
+	for (int k = 0; k < P0; ++k) {
+		const double* pk = d->phi + size_t(k) * P12;
+		ph[k] = T(pk[0]);
+		for (int m = 1; m < P12; ++m) {
+			const int ch = (m - 1) / kCH, q = (m - 1) % kCH;
+			ph[n0 + (size_t(ch) * P0 + k) * kCH + q] = T(pk[m]);
+		}
+	}
+	T* dphi = (T*)ctx->phi.get(ph.size() * sizeof(T));
+	upload(ctx, dphi, ph.data(), ph.size() * sizeof(T));
 	const int nX = div_ceil(nx, kTX), nY = div_ceil(ny, kTY);
 	const int ntiles = nX * nY;
 	const int total_steps = nt + kTX + kTY - 2;
-	int* prog = (int*)ctx->flags.get(sizeof(int) * size_t(ntiles));
-	reset_progress<<<div_ceil(ntiles, 256), 256, 0, ctx->stream>>>(prog, ntiles);
+	// flags buffer: progress counters, a zero word for out-of-range loads,
+	// and a per-tile sink for out-of-range stores
+	const size_t prog_bytes = (sizeof(int) * size_t(ntiles) + 255) / 256 * 256;
+	const size_t zero_bytes = 256;
+	unsigned char* fb = (unsigned char*)ctx->flags.get(prog_bytes + zero_bytes +
+	                                                   size_t(ntiles) * kThreads * sizeof(T));
+	int* prog = (int*)fb;
+	const T* zero = (const T*)(fb + prog_bytes);
+	T* sink = (T*)(fb + prog_bytes + zero_bytes);
+	const int nclear = int((prog_bytes + zero_bytes) / sizeof(int));
+	reset_progress<<<div_ceil(nclear, 256), 256, 0, ctx->stream>>>(prog, nclear);
 	ARMA_HIP_CHECK(hipGetLastError());
 	const int R = P1 + P2 - 1;
-	const size_t lds = size_t(R) * (kTX + P1 - 1) * (kTY + P2 - 1) * sizeof(T);
+	const size_t lds = 16 + size_t(R) * (kTX + P1 - 1) * (kTY + P2 - 1) * sizeof(T);
 	ARMA_HIP_REQUIRE(lds <= 160 * 1024, ARMA_HIP_EINVAL, "AR order too large for the LDS ring");
 	KernelTimer timer(ctx, "ar_wavefront");
 	// rings above 64 KiB (P1, P2 >= 9 in FP64) use the larger LDS of gfx950
@@ -283,7 +376,7 @@ void launch_typed(arma_hip_ctx* ctx, const arma_ar_desc* d, T* zeta) {
 				reinterpret_cast<const void*>(&ar_wave_kernel<T, P, KAHAN>),                  \
 				hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));                       \
 		ar_wave_kernel<T, P, KAHAN><<<ntiles, kThreads, lds, ctx->stream>>>(                  \
-			zeta, nt, nx, ny, dphi, P1, P2, nY, prog, total_steps);                           \
+			zeta, nt, nx, ny, dphi, dphi + n0, P1, P2, nY, prog, zero, sink, total_steps);    \
 		break;
 	switch (P0) {
 		ARMA_WAVE(1)

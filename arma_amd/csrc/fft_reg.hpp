@@ -33,7 +33,7 @@ __host__ __device__ __forceinline__ cpx<R> conj(cpx<R> a) {
 // a * (c + i s)
 template <class R>
 __host__ __device__ __forceinline__ cpx<R> cmul(cpx<R> a, R c, R s) {
-	return cpx<R>{fma(a.x, c, -(a.y * s)), fma(a.x, s, a.y * c)};
+	return cpx<R>{R(fma(a.x, c, -(a.y * s))), R(fma(a.x, s, a.y * c))};
 }
 
 // cos(2 pi q / 32), q = 0..8, correctly rounded to double.
@@ -118,6 +118,26 @@ template <int SIGN, class R>
 __device__ __forceinline__ cpx<R> twiddle(cpx<R> a, const double2* __restrict__ tw, int q) {
 	const double2 w = tw[q];
 	return cmul(a, R(w.x), R(SIGN < 0 ? w.y : -w.y));
+}
+
+// Hands the outputs of a lane's first DFT (v[brev(k2)] holds output k2)
+// to store(k2, value) after the twiddle w^(SIGN n1 k2), k2 = 0..N2-1.  Only
+// w^(n1) = tw[base] is read; its powers are formed by repeated
+// multiplication in double precision (relative error below N2 ulps).
+template <int N2, int SIGN, class R, class Store>
+__device__ __forceinline__ void twiddle_out(const cpx<R> (&v)[N2], const double2* __restrict__ tw,
+                                            int base, Store store) {
+	const double2 b = tw[base];
+	const double br = b.x, bi = SIGN < 0 ? b.y : -b.y;
+	double wr = 1.0, wi = 0.0;
+#pragma unroll
+	for (int k2 = 0; k2 < N2; ++k2) {
+		const cpx<R> a = v[brev(k2, N2)];
+		store(k2, k2 == 0 ? a : cmul(a, R(wr), R(wi)));
+		const double nr = fma(wr, br, -(wi * bi));
+		wi = fma(wr, bi, wi * br);
+		wr = nr;
+	}
 }
 
 }  // namespace arma_hip

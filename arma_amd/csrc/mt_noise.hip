@@ -262,11 +262,31 @@ __global__ __launch_bounds__(THREADS) void noise_stream_kernel(
 	const int e2 = min(lo2 + part.z, shape.z) - lo2;
 	const long long e12 = (long long)e1 * e2;
 	const long long n = (long long)(min(lo0 + part.x, shape.x) - lo0) * e12;
+	// position -> (t, x, y) inside the block; 32-bit quotients through a
+	// double reciprocal (exact after one correction) while positions fit
+	const double r12 = 1.0 / double(e12), r2 = 1.0 / double(e2);
+	const bool small = n < (1ll << 31);
 	auto put = [&](long long q, OutT v) {
-		const long long t = q / e12;
-		const long long r = q - t * e12;
-		const int x = int(r / e2);
-		const int y = int(r - (long long)x * e2);
+		long long t;
+		int x, y;
+		if (small) {
+			const uint32_t uq = uint32_t(q), u12 = uint32_t(e12), u2 = uint32_t(e2);
+			uint32_t qt = uint32_t(double(uq) * r12);
+			if (uint64_t(qt) * u12 > uq) --qt;
+			else if (uint64_t(qt + 1) * u12 <= uq) ++qt;
+			const uint32_t rr = uq - qt * u12;
+			uint32_t qx = uint32_t(double(rr) * r2);
+			if (uint64_t(qx) * u2 > rr) --qx;
+			else if (uint64_t(qx + 1) * u2 <= rr) ++qx;
+			t = qt;
+			x = int(qx);
+			y = int(rr - qx * u2);
+		} else {
+			t = q / e12;
+			const long long r = q - t * e12;
+			x = int(r / e2);
+			y = int(r - (long long)x * e2);
+		}
 		out[(size_t(lo0 + t) * shape.y + (lo1 + x)) * shape.z + (lo2 + y)] = v;
 	};
 	if (tid == 0) {
@@ -345,6 +365,161 @@ __global__ __launch_bounds__(THREADS) void noise_stream_kernel(
 	}
 }
 
+// ---- two streams per wave: one-round twist, 32 polar attempts per stream ----
+//
+// For small dcmt states (nn <= 32, 2*mm <= nn, e.g. p = 521, 607) a wave
+// runs two streams on 32 lanes each.  Every twist is a single LDS round:
+// lane k computes word k of the new state from old words only, evaluating
+// the one or two new words it depends on (k + mm - nn, and 0 and mm - 1 for
+// the last word) itself, which are themselves functions of old words when
+// 2*mm <= nn.  Each round then tempers whole twists into a word buffer until
+// 32 attempts' worth of words is there, evaluates the attempts in parallel
+// and places the accepted pairs at the positions the sequential generator
+// gives them (prefix count over the 32-lane half of the wave ballot).
+
+constexpr int kSegLanes = 32;
+
+__device__ __forceinline__ uint32_t twist_word_old(const uint32_t* st, int k,
+                                                   const arma_mt_params& p) {
+	// new word k < nn - mm: old words k, k+1, k+mm
+	return mt_twist_word(st[k], st[k + 1], st[k + p.mm], p);
+}
+
+template <class OutT, int SEM>
+__global__ __launch_bounds__(64) void noise_seg_kernel(
+	const arma_mt_params* __restrict__ params, const uint32_t* __restrict__ seeds, int3 shape,
+	int3 part, int3 np, double stddev, float stddev_f, OutT* __restrict__ out, int nstreams) {
+	constexpr int WPA = SEM == ARMA_NOISE_DOUBLE ? 4 : 2;  // words per attempt
+	constexpr int RW = kSegLanes * WPA;                     // words per round
+	__shared__ uint32_t st_all[2][32];
+	__shared__ uint32_t buf_all[2][RW + 32];
+	__shared__ int more[2];
+	const int seg = threadIdx.x >> 5;
+	const int lane = threadIdx.x & 31;
+	const int pid = blockIdx.x * 2 + seg;
+	const bool valid = pid < nstreams;
+	uint32_t* st = st_all[seg];
+	uint32_t* buf = buf_all[seg];
+	arma_mt_params p = params[valid ? pid : 0];
+	const int nn = p.nn;
+	// the block of the grid this stream fills, row-major
+	const int a = pid / (np.y * np.z), b = (pid / np.z) % np.y, c = pid % np.z;
+	const int lo0 = a * part.x, lo1 = b * part.y, lo2 = c * part.z;
+	const int e1 = valid ? min(lo1 + part.y, shape.y) - lo1 : 1;
+	const int e2 = valid ? min(lo2 + part.z, shape.z) - lo2 : 1;
+	const long long e12 = (long long)e1 * e2;
+	const long long n = valid ? (long long)(min(lo0 + part.x, shape.x) - lo0) * e12 : 0;
+	const double r12 = 1.0 / double(e12), r2 = 1.0 / double(e2);
+	auto put = [&](long long q, OutT v) {
+		// q < 2^31: 32-bit quotients through a double reciprocal, corrected
+		const uint32_t uq = uint32_t(q), u12 = uint32_t(e12), u2 = uint32_t(e2);
+		uint32_t qt = uint32_t(double(uq) * r12);
+		if (uint64_t(qt) * u12 > uq) --qt;
+		else if (uint64_t(qt + 1) * u12 <= uq) ++qt;
+		const uint32_t rr = uq - qt * u12;
+		uint32_t qx = uint32_t(double(rr) * r2);
+		if (uint64_t(qx) * u2 > rr) --qx;
+		else if (uint64_t(qx + 1) * u2 <= rr) ++qx;
+		out[(size_t(lo0 + qt) * shape.y + (lo1 + qx)) * shape.z + (lo2 + (rr - qx * u2))] = v;
+	};
+	if (lane == 0) {
+		uint32_t sd = valid && seeds ? seeds[pid] : 0u;
+		for (int k = 0; k < nn; ++k) {
+			st[k] = sd & p.wmask;
+			sd = 1812433253u * (sd ^ (sd >> 30)) + uint32_t(k + 1);
+		}
+	}
+	__syncthreads();
+	int avail = 0;
+	long long produced = 0;
+	for (;;) {
+		if (lane == 0) more[seg] = produced < n;
+		__syncthreads();
+		const bool any = more[0] || more[1];
+		__syncthreads();
+		if (!any) break;
+		// twists until a round of words is buffered (avail is the same on
+		// every lane of a segment)
+		while (true) {
+			if (lane == 0) more[seg] = avail < RW;
+			__syncthreads();
+			const bool need = more[0] || more[1];
+			const bool mine = more[seg];
+			__syncthreads();
+			if (!need) break;
+			uint32_t w = 0;
+			if (mine && lane < nn) {
+				const int k = lane, m = p.mm;
+				if (k < nn - m) {
+					w = twist_word_old(st, k, p);
+				} else if (k < nn - 1) {
+					w = mt_twist_word(st[k], st[k + 1], twist_word_old(st, k + m - nn, p), p);
+				} else {
+					w = mt_twist_word(st[nn - 1], twist_word_old(st, 0, p),
+					                  twist_word_old(st, m - 1, p), p);
+				}
+			}
+			__syncthreads();
+			if (mine && lane < nn) {
+				st[lane] = w;
+				buf[avail + lane] = mt_temper(w, p);
+			}
+			if (mine) avail += nn;
+			__syncthreads();
+		}
+		// one attempt per lane
+		const uint32_t* wv = buf + lane * WPA;
+		bool accept;
+		double xd = 0, yd = 0, r2d = 0;
+		float xf = 0, yf = 0, r2f = 0;
+		if (SEM == ARMA_NOISE_DOUBLE) {
+			xd = 2.0 * canonical_from(wv[0], wv[1]) - 1.0;
+			yd = 2.0 * canonical_from(wv[2], wv[3]) - 1.0;
+			r2d = xd * xd + yd * yd;
+			accept = !(r2d > 1.0 || r2d == 0.0);
+		} else {
+			xf = 2.0f * canonical_from_f(wv[0]) - 1.0f;
+			yf = 2.0f * canonical_from_f(wv[1]) - 1.0f;
+			r2f = xf * xf + yf * yf;
+			accept = !(r2f > 1.0f || r2f == 0.0f);
+		}
+		const unsigned long long mask = __ballot(accept);
+		const uint32_t half = uint32_t(mask >> (32 * seg));
+		const int below = __popc(half & ((1u << lane) - 1u));
+		const int total = __popc(half);
+		if (accept && produced < n) {
+			const long long pos = produced + 2ll * below;
+			if (pos < n) {
+				if (SEM == ARMA_NOISE_DOUBLE) {
+					const double mult = sqrt(-2.0 * log(r2d) / r2d);
+					put(pos, OutT(yd * mult * stddev + 0.0));
+					if (pos + 1 < n) put(pos + 1, OutT(xd * mult * stddev + 0.0));
+				} else {
+					const float mult = sqrtf(-2.0f * logf(r2f) / r2f);
+					put(pos, OutT(yf * mult * stddev_f + 0.0f));
+					if (pos + 1 < n) put(pos + 1, OutT(xf * mult * stddev_f + 0.0f));
+				}
+			}
+		}
+		produced += 2ll * total;
+		// drop the consumed words (fewer than 32 remain)
+		const int left = avail - RW;
+		const uint32_t keep = lane < left ? buf[RW + lane] : 0u;
+		__syncthreads();
+		if (lane < left) buf[lane] = keep;
+		avail = left;
+		__syncthreads();
+	}
+}
+
+// The two-per-wave kernel applies to states of at most 32 words whose twist
+// needs only old words for its redundant evaluations.
+bool seg_kernel_fits(const arma_mt_params* mts, int n) {
+	for (int i = 0; i < n; ++i)
+		if (mts[i].nn > kSegLanes || 2 * mts[i].mm > mts[i].nn) return false;
+	return true;
+}
+
 int threads_for_state(int nn) {
 	// per-thread LDS state of nn words; at most 64 KiB per workgroup
 	int t = 256;
@@ -420,6 +595,24 @@ void launch_white_noise(arma_hip_ctx* ctx, const arma_ar_desc* d, void* eps) {
 		upload(ctx, dseeds, d->seeds, sizeof(uint32_t) * ntotal);
 	}
 	KernelTimer timer(ctx, "noise_partitioned");
+	// every stream's block must fit 32-bit positions for the fast kernels
+	const bool small_blocks = (long long)part.x * part.y * part.z < (1ll << 31);
+	if (small_blocks && seg_kernel_fits(d->mts, ntotal)) {
+		// two streams per wave
+#define ARMA_LAUNCH_SEG(OT, S)                                                                   \
+	noise_seg_kernel<OT, S><<<div_ceil(ntotal, 2), 64, 0, ctx->stream>>>(                        \
+		dparams, dseeds, shape, part, np, stddev, stddev_f, (OT*)eps, ntotal)
+		if (fp32) {
+			if (sem == ARMA_NOISE_DOUBLE) ARMA_LAUNCH_SEG(float, ARMA_NOISE_DOUBLE);
+			else ARMA_LAUNCH_SEG(float, ARMA_NOISE_FLOAT);
+		} else {
+			if (sem == ARMA_NOISE_DOUBLE) ARMA_LAUNCH_SEG(double, ARMA_NOISE_DOUBLE);
+			else ARMA_LAUNCH_SEG(double, ARMA_NOISE_FLOAT);
+		}
+#undef ARMA_LAUNCH_SEG
+		ARMA_HIP_CHECK(hipGetLastError());
+		return;
+	}
 	if (nn_max <= 64) {
 		// one wave per stream (dcmt exponents up to 1279)
 #define ARMA_LAUNCH_WAVE(OT, S)                                                                  \

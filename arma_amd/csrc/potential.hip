@@ -131,11 +131,8 @@ __global__ __launch_bounds__(kThreads) void fwd_rows_kernel(
 			}
 		}
 		dft<N2, -1>(v);
-#pragma unroll
-		for (int m = 0; m < N2; ++m) {
-			const int k2 = brev(m, N2);
-			X[xidx<N1, N2>(r, k2, n1)] = twiddle<-1>(v[m], twy, 2 * n1 * k2);
-		}
+		twiddle_out<N2, -1>(v, twy, 2 * n1,
+		                    [&](int k2, cpx<T> a) { X[xidx<N1, N2>(r, k2, n1)] = a; });
 	}
 	__syncthreads();
 	// second DFTs (at most two tasks per lane: R*N2 <= 2*R*N1 <= 512)
@@ -196,11 +193,8 @@ __global__ __launch_bounds__(kThreads) void fwd_cols_kernel(int nx, int M,
 		for (int n2 = 0; n2 < N2; ++n2)
 			v[n2] = col <= M ? base[size_t(col) * nx + n1 + N1 * n2] : cpx<T>{T(0), T(0)};
 		dft<N2, -1>(v);
-#pragma unroll
-		for (int m = 0; m < N2; ++m) {
-			const int k2 = brev(m, N2);
-			X[xidx<N1, N2>(s, k2, n1)] = twiddle<-1>(v[m], twx, n1 * k2);
-		}
+		twiddle_out<N2, -1>(v, twx, n1,
+		                    [&](int k2, cpx<T> a) { X[xidx<N1, N2>(s, k2, n1)] = a; });
 	}
 	__syncthreads();
 	for (int t = tid; t < S * N2; t += kThreads) {
@@ -243,46 +237,41 @@ __device__ __forceinline__ Ev first_max(const Ev& cur, const Ev& e) {
 	return fabs(cur.elev) < fabs(e.elev) ? e : cur;
 }
 
+// Columns of the row tile staged in LDS by the row scan.
+__host__ __device__ constexpr int fw_tile(int lines) { return lines > 512 ? 8 : 16; }
+
+// One block per (slice, direction) with one thread per scanned line: thread
+// b scans line b (dir 0: column y = b along x; dir 1: row x = b along y), so
+// its positions are exactly the contiguous chunk [b*inner, (b+1)*inner) of
+// the sequential scan order.  Column scans read coalesced rows of the slice;
+// row scans go through LDS tiles of the slice's columns.
 template <class T>
-__global__ __launch_bounds__(256) void factor_waves_kernel(const T* __restrict__ zeta, int nx,
-                                                          int ny, const int* __restrict__ idx_t,
-                                                          const int* __restrict__ t_int,
-                                                          double dt, double* __restrict__ kout) {
-	__shared__ Summary sm[256];
+__global__ __launch_bounds__(1024) void factor_waves_kernel(const T* __restrict__ zeta, int nx,
+                                                           int ny, const int* __restrict__ idx_t,
+                                                           const int* __restrict__ t_int,
+                                                           double dt, double* __restrict__ kout) {
+	extern __shared__ __align__(16) unsigned char fw_smem[];
+	Summary* sm = reinterpret_cast<Summary*>(fw_smem);  // after the scan
+	double* tile = reinterpret_cast<double*>(fw_smem);  // during a row scan
 	const int i = blockIdx.x;
 	const int dir = blockIdx.y;
 	const T* z = zeta + size_t(idx_t[i]) * nx * ny;
 	const double t = double(t_int[i]);
-	// view: Z(a, b) with a scanned (1..n0-2) inside b (0..n1-1)
-	const int n0 = dir == 0 ? nx : ny;
-	const int n1 = dir == 0 ? ny : nx;
-	const int inner = n0 - 2;
-	const long long P = inner > 0 ? (long long)inner * n1 : 0;
-	const long long L = (P + blockDim.x - 1) / blockDim.x;
-	const long long p0 = (long long)threadIdx.x * L;
-	const long long p1 = min(P, p0 + L);
+	const int n0 = dir == 0 ? nx : ny;  // scanned extent
+	const int n1 = dir == 0 ? ny : nx;  // lines, one per thread
+	const int b = threadIdx.x;
+	const bool has_line = b < n1;  // the block spans the larger line count
 	Summary s;
 	s.runs = 0;
 	s.kmin = 1.7976931348623157e308;
 	s.kmax = 2.2250738585072014e-308;
 	Ev cur{0, 0}, prev{0, 0};
 	int cur_pos = 0;
-	for (long long p = p0; p < p1; ++p) {
-		const int b = int(p / inner);
-		const int a = int(p - (long long)b * inner) + 1;
-		double z0, z1, z2;
-		if (dir == 0) {
-			z0 = double(z[size_t(a) * ny + b]);
-			z1 = double(z[size_t(a - 1) * ny + b]);
-			z2 = double(z[size_t(a + 1) * ny + b]);
-		} else {
-			z0 = double(z[size_t(b) * ny + a]);
-			z1 = double(z[size_t(b) * ny + a - 1]);
-			z2 = double(z[size_t(b) * ny + a + 1]);
-		}
+	// the event test and run update at one position (z1, z0, z2 = a-1, a, a+1)
+	auto visit = [&](double z1, double z0, double z2) {
 		const double dw1 = z0 - z1;
 		const double dw2 = z0 - z2;
-		if ((dw1 > 0.0) != (dw2 > 0.0)) continue;
+		if ((dw1 > 0.0) != (dw2 > 0.0)) return;
 		const double aa = -0.5 * (dw1 + dw2) / (dt * dt);
 		const double bb = dw1 / dt - aa * dt * (2.0 * t - 1.0);
 		const double cc = z0 - t * dt * (aa * t * dt + bb);
@@ -294,7 +283,7 @@ __global__ __launch_bounds__(256) void factor_waves_kernel(const T* __restrict__
 			s.first_pos = pos;
 			cur = e;
 			cur_pos = pos;
-			continue;
+			return;
 		}
 		if (pos == cur_pos) {
 			cur = first_max(cur, e);
@@ -313,6 +302,37 @@ __global__ __launch_bounds__(256) void factor_waves_kernel(const T* __restrict__
 			cur_pos = pos;
 			++s.runs;
 		}
+	};
+	if (dir == 0) {
+		if (n0 >= 3 && has_line) {
+			double zm = double(z[b]), zc = double(z[size_t(ny) + b]);
+			for (int a = 1; a + 1 < n0; ++a) {
+				const double zp = double(z[size_t(a + 1) * ny + b]);
+				visit(zm, zc, zp);
+				zm = zc;
+				zc = zp;
+			}
+		}
+	} else {
+		const int TW = fw_tile(n1);
+		double zm = 0.0, zc = 0.0;
+		for (int c0 = 0; c0 < n0; c0 += TW) {
+			__syncthreads();
+			for (int e = b; e < n1 * TW; e += blockDim.x) {
+				const int row = e / TW, col = e - row * TW;
+				if (c0 + col < n0) tile[row * (TW + 1) + col] = double(z[size_t(row) * ny + c0 + col]);
+			}
+			__syncthreads();
+			const int cw = has_line ? min(TW, n0 - c0) : 0;
+			for (int col = 0; col < cw; ++col) {
+				const double zp = tile[b * (TW + 1) + col];
+				const int c = c0 + col;  // position of zp along the row
+				if (c >= 2) visit(zm, zc, zp);
+				zm = zc;
+				zc = zp;
+			}
+		}
+		__syncthreads();  // the tile area becomes the summary array
 	}
 	if (s.runs == 1) s.first = cur;
 	if (s.runs == 2) s.second = cur;
@@ -337,7 +357,7 @@ __global__ __launch_bounds__(256) void factor_waves_kernel(const T* __restrict__
 			cls = ncls;
 			++j;
 		};
-		for (int c = 0; c < blockDim.x; ++c) {
+		for (int c = 0; c < n1; ++c) {
 			const Summary& q = sm[c];
 			if (q.runs == 0) continue;
 			if (!have) {
@@ -431,22 +451,18 @@ struct WinParts {
 
 // ---- inverse transforms -------------------------------------------------------
 
-// Packed columns c0..c0+CB-1 of slice i (column 0 carries ky = 0 and ky = M),
-// times W_sym(z) for every level, inverse transform of length nx = N1*N2
-// along x.  Output G[(il, z)][c][x].  Levels go in groups of ZB through LDS;
-// ZB*CB*N1 <= 256.
+// Column task: packed columns c0..c0+CB-1 of slice i (column 0 carries
+// ky = 0 and ky = M), times W_sym(z) for every level, inverse transform of
+// length nx = N1*N2 along x.  Output Gs[z][c][x].  Levels go in groups of ZB
+// through LDS; ZB*CB*N1 <= 256.
 template <class T, int N1, int N2>
-__global__ __launch_bounds__(kThreads) void inv_cols_kernel(
-	int ny, int M, int nz, int i0, const double2* __restrict__ twx,
-	const cpx<T>* __restrict__ spec, const double* __restrict__ kmin, double z0, double dz,
-	double h, cpx<T>* __restrict__ G, int CB, int ZB) {
+__device__ __forceinline__ void col_task(cpx<T>* X, int ny, int M, int nz, int i, int c0,
+                                         const double2* __restrict__ twx,
+                                         const cpx<T>* __restrict__ spec,
+                                         const double* __restrict__ kmin, double z0, double dz,
+                                         double h, cpx<T>* Gs, int CB, int ZB) {
 	constexpr int N = N1 * N2;
 	constexpr int EPT = N > kThreads ? N / kThreads : 1;  // CB*N <= max(256, N)
-	extern __shared__ unsigned char smem_raw[];
-	cpx<T>* X = reinterpret_cast<cpx<T>*>(smem_raw);
-	const int il = blockIdx.y;
-	const int i = i0 + il;
-	const int c0 = blockIdx.x * CB;
 	const int tid = threadIdx.x;
 	const int nel = CB * N;
 	const cpx<T>* base = spec + size_t(i) * (M + 1) * N;
@@ -512,11 +528,8 @@ __global__ __launch_bounds__(kThreads) void inv_cols_kernel(
 		__syncthreads();
 		if (act) {
 			dft<N2, +1>(v);
-#pragma unroll
-			for (int m = 0; m < N2; ++m) {
-				const int k2 = brev(m, N2);
-				X[xidx<N1, N2>(s, k2, n1)] = twiddle<+1>(v[m], twx, n1 * k2);
-			}
+			twiddle_out<N2, +1>(v, twx, n1,
+			                    [&](int k2, cpx<T> a) { X[xidx<N1, N2>(s, k2, n1)] = a; });
 		}
 		__syncthreads();
 		for (int t = tid; t < nseq * N2; t += kThreads) {
@@ -526,7 +539,7 @@ __global__ __launch_bounds__(kThreads) void inv_cols_kernel(
 #pragma unroll
 			for (int m = 0; m < N1; ++m) w[m] = X[xidx<N1, N2>(sq, k2, m)];
 			dft<N1, +1>(w);
-			cpx<T>* out = G + ((size_t(il) * nz + j0 + zz) * M + c0 + c) * N;
+			cpx<T>* out = Gs + (size_t(j0 + zz) * M + c0 + c) * N;
 #pragma unroll
 			for (int m = 0; m < N1; ++m) out[k2 + N2 * brev(m, N1)] = w[m];
 		}
@@ -534,20 +547,14 @@ __global__ __launch_bounds__(kThreads) void inv_cols_kernel(
 	}
 }
 
-// Rows: real output of length ny = 2M from the M packed columns, through a
-// complex transform of length M = N1*N2, scaled by 1/(nx ny).  R rows per
-// block, R*N1 <= 256.
+// Row task: rows x0..x0+R-1 of one (slice, level): real output of length
+// ny = 2M from the M packed columns g[k][x], through a complex transform of
+// length M = N1*N2, scaled by 1/(nx ny).  R*N1 <= 256.
 template <class T, int N1, int N2>
-__global__ __launch_bounds__(kThreads) void inv_rows_kernel(int nx, int ny, int nz, int i0,
-                                                             const double2* __restrict__ twy,
-                                                             const cpx<T>* __restrict__ G,
-                                                             T* __restrict__ phi, int R) {
+__device__ __forceinline__ void row_task(cpx<T>* X, int nx, int ny, int x0,
+                                         const double2* __restrict__ twy, const cpx<T>* g,
+                                         T* __restrict__ out, int R) {
 	constexpr int M = N1 * N2;
-	extern __shared__ unsigned char smem_raw[];
-	cpx<T>* X = reinterpret_cast<cpx<T>*>(smem_raw);
-	const int item = blockIdx.y;  // (il, z)
-	const int x0 = blockIdx.x * R;
-	const cpx<T>* g = G + size_t(item) * M * nx;
 	const int tid = threadIdx.x;
 	if (tid < R * N1) {
 		const int r = tid % R, n1 = tid / R;
@@ -573,15 +580,10 @@ __global__ __launch_bounds__(kThreads) void inv_rows_kernel(int nx, int ny, int 
 			}
 		}
 		dft<N2, +1>(v);
-#pragma unroll
-		for (int m = 0; m < N2; ++m) {
-			const int k2 = brev(m, N2);
-			X[xidx<N1, N2>(r, k2, n1)] = twiddle<+1>(v[m], twy, 2 * n1 * k2);
-		}
+		twiddle_out<N2, +1>(v, twy, 2 * n1,
+		                    [&](int k2, cpx<T> a) { X[xidx<N1, N2>(r, k2, n1)] = a; });
 	}
 	__syncthreads();
-	const int il = item / nz, j = item - il * nz;
-	T* out = phi + ((size_t(i0 + il) * nz + j) * nx) * ny;
 	const T inv = T(1.0 / (double(nx) * double(ny)));
 	for (int t = tid; t < R * N2; t += kThreads) {
 		const int k2 = t % N2, r = t / N2;
@@ -599,6 +601,35 @@ __global__ __launch_bounds__(kThreads) void inv_rows_kernel(int nx, int ny, int 
 			}
 		}
 	}
+}
+
+// Column pass over a chunk of output slices: block (cb, il) runs the column
+// task of slice i0 + il, columns cb*CB...; G holds [il][z][c][x].
+template <class T, int N1, int N2>
+__global__ __launch_bounds__(kThreads) void inv_cols_kernel(
+	int ny, int M, int nz, int i0, const double2* __restrict__ twx,
+	const cpx<T>* __restrict__ spec, const double* __restrict__ kmin, double z0, double dz,
+	double h, cpx<T>* __restrict__ G, int CB, int ZB) {
+	extern __shared__ unsigned char smem_raw[];
+	cpx<T>* X = reinterpret_cast<cpx<T>*>(smem_raw);
+	const int il = blockIdx.y;
+	col_task<T, N1, N2>(X, ny, M, nz, i0 + il, blockIdx.x * CB, twx, spec, kmin, z0, dz, h,
+	                    G + size_t(il) * nz * M * (N1 * N2), CB, ZB);
+}
+
+// Row pass over a chunk: block (rb, item) runs the row task of
+// (slice i0 + item / nz, level item % nz), rows rb*R...
+template <class T, int N1, int N2>
+__global__ __launch_bounds__(kThreads) void inv_rows_kernel(int nx, int ny, int nz, int i0,
+                                                             const double2* __restrict__ twy,
+                                                             const cpx<T>* __restrict__ G,
+                                                             T* __restrict__ phi, int R) {
+	extern __shared__ unsigned char smem_raw[];
+	cpx<T>* X = reinterpret_cast<cpx<T>*>(smem_raw);
+	const int item = blockIdx.y;
+	const int il = item / nz, j = item - il * nz;
+	row_task<T, N1, N2>(X, nx, ny, blockIdx.x * R, twy, G + size_t(item) * (N1 * N2) * nx,
+	                    phi + ((size_t(i0 + il) * nz + j) * nx) * ny, R);
 }
 
 // Plane through three grid points evaluated at a fourth
@@ -668,18 +699,6 @@ struct ColLaunch {
 		ARMA_HIP_CHECK(hipGetLastError());
 	}
 
-	static void inverse(arma_hip_ctx* ctx, const PotPlan& p, int i0, int cn, const double2* twx,
-	                    const cpx<T>* spec, const double* dk, cpx<T>* G) {
-		const int CB = std::max(1, std::min(p.M, kThreads / NX));
-		const size_t per_seq = sizeof(cpx<T>) * size_t(std::max(NX, xsize<N1, N2>(1)));
-		int ZB = std::min(p.nz, std::max(1, kThreads / N1 / CB));
-		while (ZB > 1 && size_t(ZB) * CB * per_seq > 81920) --ZB;
-		const size_t lds = size_t(ZB) * CB * per_seq;
-		allow_lds(&inv_cols_kernel<T, N1, N2>, lds);
-		inv_cols_kernel<T, N1, N2><<<dim3(div_ceil(p.M, CB), cn), kThreads, lds, ctx->stream>>>(
-			p.ny, p.M, p.nz, i0, twx, spec, dk, p.z0, p.dz, p.h, G, CB, ZB);
-		ARMA_HIP_CHECK(hipGetLastError());
-	}
 };
 
 template <class T, int M>
@@ -699,18 +718,83 @@ struct RowLaunch {
 		ARMA_HIP_CHECK(hipGetLastError());
 	}
 
-	static void inverse(arma_hip_ctx* ctx, const PotPlan& p, int i0, int cn, const double2* twy,
-	                    const cpx<T>* G, T* phi) {
-		const int R = rows(p);
-		const size_t lds = sizeof(cpx<T>) * size_t(xsize<N1, N2>(R));
-		allow_lds(&inv_rows_kernel<T, N1, N2>, lds);
-		inv_rows_kernel<T, N1, N2><<<dim3(div_ceil(p.nx, R), cn * p.nz), kThreads, lds,
-		                             ctx->stream>>>(p.nx, p.ny, p.nz, i0, twy, G, phi, R);
-		ARMA_HIP_CHECK(hipGetLastError());
-	}
 };
 
-// Calls F<L>::run(args...) for the power of two L = n in [lo, 1024].
+// Bytes of the column-transformed intermediate per chunk of output slices
+// (ARMA_HIP_POT_CHUNK_MB overrides).
+size_t chunk_bytes() {
+	if (const char* e = std::getenv("ARMA_HIP_POT_CHUNK_MB")) {
+		const long v = std::atol(e);
+		if (v > 0) return size_t(v) << 20;
+	}
+	return size_t(8) << 30;
+}
+
+// Column and row passes for nx = NX and ny = 2M over chunks of output slices.
+template <class T, int NX, int M>
+void launch_inverse(arma_hip_ctx* ctx, const PotPlan& p, const double2* twx, const double2* twy,
+                    const cpx<T>* spec, const double* dk, T* phi) {
+	constexpr int CN1 = split1(NX), CN2 = NX / CN1, RN1 = split1(M), RN2 = M / RN1;
+	const int CB = std::max(1, std::min(M, kThreads / NX));
+	const size_t per_seq = sizeof(cpx<T>) * size_t(std::max(NX, xsize<CN1, CN2>(1)));
+	int ZB = std::min(p.nz, std::max(1, kThreads / CN1 / CB));
+	while (ZB > 1 && size_t(ZB) * CB * per_seq > 81920) --ZB;
+	const size_t lds_c = size_t(ZB) * CB * per_seq;
+	const int R = std::min(NX, kThreads / RN1);
+	const size_t lds_r = sizeof(cpx<T>) * size_t(xsize<RN1, RN2>(R));
+	allow_lds(&inv_cols_kernel<T, CN1, CN2>, lds_c);
+	allow_lds(&inv_rows_kernel<T, RN1, RN2>, lds_r);
+	// the intermediate of a whole chunk round-trips through HBM; chunks are
+	// large so that the launch count stays small
+	const size_t per_slice = sizeof(cpx<T>) * size_t(p.nz) * M * NX;
+	const int chunk = std::min(p.nto, int(std::max<size_t>(1, chunk_bytes() / per_slice)));
+	cpx<T>* G = (cpx<T>*)ctx->pot_mid.get(per_slice * chunk);
+	for (int i0 = 0; i0 < p.nto; i0 += chunk) {
+		const int cn = std::min(chunk, p.nto - i0);
+		{
+			KernelTimer t(ctx, "pot_inv_cols");
+			inv_cols_kernel<T, CN1, CN2><<<dim3(M / CB, cn), kThreads, lds_c, ctx->stream>>>(
+				p.ny, M, p.nz, i0, twx, spec, dk, p.z0, p.dz, p.h, G, CB, ZB);
+			ARMA_HIP_CHECK(hipGetLastError());
+		}
+		{
+			KernelTimer t(ctx, "pot_inv_rows");
+			inv_rows_kernel<T, RN1, RN2><<<dim3(div_ceil(NX, R), cn * p.nz), kThreads, lds_r,
+			                               ctx->stream>>>(NX, 2 * M, p.nz, i0, twy, G, phi, R);
+			ARMA_HIP_CHECK(hipGetLastError());
+		}
+	}
+}
+
+template <class T, int NX, int M = 2>
+void inverse_by_m(int m, arma_hip_ctx* ctx, const PotPlan& p, const double2* twx,
+                  const double2* twy, const cpx<T>* spec, const double* dk, T* phi) {
+	if constexpr (M <= 512) {
+		if (m == M) {
+			launch_inverse<T, NX, M>(ctx, p, twx, twy, spec, dk, phi);
+			return;
+		}
+		inverse_by_m<T, NX, M * 2>(m, ctx, p, twx, twy, spec, dk, phi);
+	} else {
+		ARMA_HIP_REQUIRE(false, ARMA_HIP_EINVAL, "transform length must be a power of two");
+	}
+}
+
+template <class T, int NX = 4>
+void inverse_dispatch(int nx, int m, arma_hip_ctx* ctx, const PotPlan& p, const double2* twx,
+                      const double2* twy, const cpx<T>* spec, const double* dk, T* phi) {
+	if constexpr (NX <= 1024) {
+		if (nx == NX) {
+			inverse_by_m<T, NX>(m, ctx, p, twx, twy, spec, dk, phi);
+			return;
+		}
+		inverse_dispatch<T, NX * 2>(nx, m, ctx, p, twx, twy, spec, dk, phi);
+	} else {
+		ARMA_HIP_REQUIRE(false, ARMA_HIP_EINVAL, "transform length must be a power of two");
+	}
+}
+
+// Calls F<L>::run(args...) for the power of two L = n in [2, 1024].
 template <template <int> class F, int L = 2, class... A>
 void by_length(int n, A&&... args) {
 	if constexpr (L <= 1024) {
@@ -733,37 +817,13 @@ struct Ops {
 		}
 	};
 	template <int L>
-	struct ColInv {
-		static void run(arma_hip_ctx* c, const PotPlan& p, int i0, int cn, const double2* tw,
-		                const cpx<T>* s, const double* dk, cpx<T>* G) {
-			if constexpr (L >= 4) ColLaunch<T, L>::inverse(c, p, i0, cn, tw, s, dk, G);
-		}
-	};
-	template <int L>
 	struct RowFwd {
 		static void run(arma_hip_ctx* c, const PotPlan& p, const T* z, const int* di,
 		                const double2* tw, cpx<T>* s) {
 			if constexpr (L <= 512) RowLaunch<T, L>::forward(c, p, z, di, tw, s);
 		}
 	};
-	template <int L>
-	struct RowInv {
-		static void run(arma_hip_ctx* c, const PotPlan& p, int i0, int cn, const double2* tw,
-		                const cpx<T>* G, T* phi) {
-			if constexpr (L <= 512) RowLaunch<T, L>::inverse(c, p, i0, cn, tw, G, phi);
-		}
-	};
 };
-
-// Bytes of the inverse intermediate per pass: small enough to stay in the
-// last-level cache between the two passes.  ARMA_HIP_POT_CHUNK_MB overrides.
-size_t chunk_bytes() {
-	if (const char* e = std::getenv("ARMA_HIP_POT_CHUNK_MB")) {
-		const long v = std::atol(e);
-		if (v > 0) return size_t(v) << 20;
-	}
-	return size_t(128) << 20;
-}
 
 template <class T>
 void run_potential(arma_hip_ctx* ctx, const arma_pot_desc* d, const T* zeta, T* phi,
@@ -805,7 +865,13 @@ void run_potential(arma_hip_ctx* ctx, const arma_pot_desc* d, const T* zeta, T* 
 	double* dk = (double*)ctx->pot_kmin.get(sizeof(double) * 4 * size_t(p.nto));
 	{
 		KernelTimer t_fw(ctx, "pot_factor_waves");
-		factor_waves_kernel<T><<<dim3(p.nto, 2), 256, 0, st>>>(zeta, p.nx, p.ny, didx, dtint,
+		const int lines = std::max(p.nx, p.ny);
+		const size_t fw_lds = std::max(sizeof(Summary) * size_t(lines),
+		                               sizeof(double) * size_t(p.nx) * (fw_tile(p.nx) + 1));
+		allow_lds(&factor_waves_kernel<T>, fw_lds);
+		// both directions in one launch: the block size is the larger line count and
+		// threads past a direction's line count only take part in barriers
+		factor_waves_kernel<T><<<dim3(p.nto, 2), lines, fw_lds, st>>>(zeta, p.nx, p.ny, didx, dtint,
 		                                                       d->delta[0], dk);
 		ARMA_HIP_CHECK(hipGetLastError());
 	}
@@ -817,25 +883,12 @@ void run_potential(arma_hip_ctx* ctx, const arma_pot_desc* d, const T* zeta, T* 
 		                                   (const double2*)twy, spec);
 		by_length<Ops<T>::template ColFwd>(p.nx, ctx, p, (const double2*)twx, spec);
 	}
-	// inverse transforms in chunks of output slices
-	const size_t per_slice = size_t(p.nz) * p.M * p.nx * sizeof(cpx<T>);
-	int chunk = int(std::max<size_t>(1, chunk_bytes() / per_slice));
-	chunk = std::min(chunk, p.nto);
-	cpx<T>* G = (cpx<T>*)ctx->pot_mid.get(per_slice * chunk);
-	for (int i0 = 0; i0 < p.nto; i0 += chunk) {
-		const int cn = std::min(chunk, p.nto - i0);
-		{
-			KernelTimer t_ic(ctx, "pot_inv_cols");
-			by_length<Ops<T>::template ColInv>(p.nx, ctx, p, i0, cn, (const double2*)twx,
-			                                   (const cpx<T>*)spec, (const double*)dk, G);
-		}
-		{
-			KernelTimer t_ir(ctx, "pot_inv_rows");
-			by_length<Ops<T>::template RowInv>(p.M, ctx, p, i0, cn, (const double2*)twy,
-			                                   (const cpx<T>*)G, phi);
-		}
+	// every (slice, level): window, inverse along x, inverse along y
+	inverse_dispatch<T>(p.nx, p.M, ctx, p, (const double2*)twx, (const double2*)twy,
+	                    (const cpx<T>*)spec, (const double*)dk, phi);
+	{
 		KernelTimer t_ed(ctx, "pot_edge");
-		edge_kernel<T><<<cn * p.nz, 256, 0, st>>>(p.nx, p.ny, phi, i0, p.nz);
+		edge_kernel<T><<<p.nto * p.nz, 256, 0, st>>>(p.nx, p.ny, phi, 0, p.nz);
 		ARMA_HIP_CHECK(hipGetLastError());
 	}
 	if (wnmax_out) {

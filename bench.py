@@ -64,42 +64,63 @@ def setup_model(arma, w):
     return acf, phi, varwn
 
 
-def cpu_baseline(w, phi, varwn, params):
-    """Oracle timings on a bounded sample: the first slices of the surface and
-    a few time slices of the potential.  Single-threaded."""
+def sample_shape(w):
+    """The bounded sample the CPU oracle runs: the first slices of the surface
+    (two partition layers, or the whole surface when small) and the potential
+    of the first few of them."""
+    nt, nx, ny = w["shape"]
+    ts = min(nt, 2 * w["partition"][0] if nx * ny >= 65536 else nt)
+    return ts, 4
+
+
+def errors(got, ref):
+    import numpy as np
+    got = np.asarray(got, np.float64)
+    ref = np.asarray(ref, np.float64)
+    d = got - ref
+    return {"max_rel": float(np.max(np.abs(d)) / np.max(np.abs(ref))),
+            "rms_rel": float(np.sqrt(np.mean(d * d) / np.mean(ref * ref)))}
+
+
+def cpu_baseline(w, phi, varwn, params, seeds, gpu_sample):
+    """Oracle timings on a bounded sample: the first slices of the surface
+    (causal, so they depend on the first slices of noise only) and a few time
+    slices of the potential.  Single-threaded.  The same oracle output is the
+    FP64 reference for the GPU's output on this sample (same seeds, before the
+    post-rescale, whose factor depends on the whole surface)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import pyoracle as O
     nt, nx, ny = w["shape"]
     part = w["partition"]
-    ts = min(nt, 2 * part[0] if nx * ny >= 65536 else nt)
+    ts, tp = sample_shape(w)
     sample = (ts, nx, ny)
     np_ = [math.ceil(sample[d] / part[d]) for d in range(3)]
     nstreams = np_[0] * np_[1] * np_[2]
     plist = [O.MtParams(*params[i].as_tuple()) for i in range(nstreams)]
     t0 = time.perf_counter()
-    eps = O.noise_partitioned(plist, None, sample, part, math.sqrt(varwn))
+    eps = O.noise_partitioned(plist, seeds[:nstreams], sample, part, math.sqrt(varwn))
     zeta = O.ar_generate(eps, phi)
     t_surf = time.perf_counter() - t0
     surf_pts = ts * nx * ny
     res = {"surface_pts_per_s": surf_pts / t_surf}
     desc = f"noise+AR on {ts}x{nx}x{ny}"
+    acc = {"surface": errors(gpu_sample["zeta"], zeta)}
     if w.get("depth") is not None:
         nz = w["npoints"][1]
-        tp = 4
-        z = O.ar_generate(O.noise_partitioned(plist, None, sample, part, math.sqrt(varwn)), phi)
-        z = np.nan_to_num(z[:max(tp, 3)], nan=0.0, posinf=0.0, neginf=0.0)
         t0 = time.perf_counter()
-        O.linear_potential(z, (2 * math.pi,) * 3, w["depth"], (0.0, w["lbound"][1]),
-                           (float(tp - 1), w["ubound"][1]), (tp, nz))
+        pot = O.linear_potential(zeta, (2 * math.pi,) * 3, w["depth"], (0.0, w["lbound"][1]),
+                                 (float(tp - 1), w["ubound"][1]), (tp, nz))
         t_pot = time.perf_counter() - t0
         res["potential_pts_per_s"] = tp * nz * nx * ny / t_pot
         desc += f"; potential of {tp} time slices x {nz} z levels"
         value = (surf_pts + tp * nz * nx * ny) / (t_surf + t_pot)
+        acc["potential"] = errors(gpu_sample["potential"], pot)
     else:
         value = res["surface_pts_per_s"]
-    del zeta
-    return {"value": value, "unit": "pts/s", "cores": 1, "kind": "port", "sample": desc, **res}
+    acc["against"] = "FP64 CPU oracle, same seeds, on the sample"
+    return {"value": value, "unit": "pts/s", "cores": 1, "kind": "port", "sample": desc, **res,
+            "accuracy": acc}
 
 
 def parse_profile(ctx):
@@ -240,9 +261,20 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
+        # the GPU's own output on the sample (untimed): surface before the
+        # post-rescale, and the potential of the first time slices
+        ts, tp = sample_shape(w)
+        ctx.ar_generate(desc, zeta)
+        gpu_sample = {"zeta": zeta[:ts].double().cpu().numpy()}
+        if has_pot:
+            sdesc = ctx.pot_desc(w["shape"], (2 * math.pi,) * 3, w["depth"],
+                                 (0.0, w["lbound"][1]), (float(tp - 1), w["ubound"][1]),
+                                 (tp, nz), prec)
+            ctx.linear_potential(sdesc, zeta, phi_out)
+            gpu_sample["potential"] = phi_out[:tp].double().cpu().numpy()
         log("bench: timing the CPU oracle on a sample")
         try:
-            cpu = cpu_baseline(w, phi, varwn, params)
+            cpu = cpu_baseline(w, phi, varwn, params, seeds, gpu_sample)
         except Exception as e:  # report, never fail the bench line for it
             cpu = {"error": str(e)}
 

@@ -81,6 +81,44 @@ def test_partitioned_noise(hip_ctx, p521_params):
     assert ulp_diff(eps.cpu().numpy(), ref) <= 4
 
 
+@pytest.mark.parametrize("sem", ["double", "float"])
+def test_partitioned_noise_odd_stream_count(hip_ctx, p521_params, sem):
+    """27 ragged partitions (an odd count leaves half of the last wave idle);
+    each partition is one sequential stream, in double or float semantics."""
+    shape, part = (9, 10, 9), (4, 4, 4)
+    npart = [-(-shape[k] // part[k]) for k in range(3)]
+    nparts = npart[0] * npart[1] * npart[2]
+    plist = [p521_params[i] for i in range(nparts)]
+    seeds = np.arange(nparts, dtype=np.uint32) * 3 + 5
+    fp32 = sem == "float"
+    d = hip_ctx.ar_desc(shape, np.zeros((1, 1, 1)), 0.9, plist, seeds=seeds,
+                        noise_mode=arma_amd.NOISE_PARTITIONED, partition=part,
+                        precision=arma_amd.FP32 if fp32 else arma_amd.FP64,
+                        noise_semantics=arma_amd._abi.NOISE_FLOAT if fp32 else arma_amd._abi.NOISE_DOUBLE)
+    eps = torch.empty(shape, dtype=torch.float32 if fp32 else torch.float64, device=DEV)
+    hip_ctx.white_noise(d, eps)
+    dt = np.float32 if fp32 else np.float64
+    sigma = np.float32(np.sqrt(np.float32(0.9))) if fp32 else np.sqrt(0.9)
+    ref = np.empty(shape, dt)
+    for a in range(npart[0]):
+        for b in range(npart[1]):
+            for c in range(npart[2]):
+                pid = (a * npart[1] + b) * npart[2] + c
+                lo = (a * part[0], b * part[1], c * part[2])
+                hi = [min(lo[k] + part[k], shape[k]) for k in range(3)]
+                ext = [hi[k] - lo[k] for k in range(3)]
+                vals = O.noise_sequential(to_oracle(plist[pid]), int(seeds[pid]), sigma,
+                                          ext[0] * ext[1] * ext[2], dtype=dt)
+                ref[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] = vals.reshape(ext)
+    got = eps.cpu().numpy()
+    if fp32:
+        # ulps of float32; device logf and glibc logf differ in the last place at times
+        ulps = np.abs(got - ref) / np.spacing(np.maximum(np.abs(got), np.abs(ref)))
+        assert float(np.max(ulps)) <= 4 and float(np.mean(got == ref)) > 0.5
+    else:
+        assert ulp_diff(got, ref) <= 4
+
+
 def test_noise_errors(hip_ctx, p521_params):
     d = hip_ctx.ar_desc((8, 8, 8), np.zeros((1, 1, 1)), 1.0, [p521_params[0]],
                         noise_mode=arma_amd.NOISE_PARTITIONED, partition=(4, 4, 4))
