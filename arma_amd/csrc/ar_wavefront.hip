@@ -44,6 +44,7 @@ constexpr int kTY = 16;
 constexpr int kThreads = kTX * kTY;
 constexpr int kD = 4;  // prefetch distance of the noise and halo loads, in steps
 constexpr int kMaxP12 = 10;  // largest P1, P2 handled
+constexpr int kProgStride = 32;  // ints between tile progress counters: one per 128-byte line
 
 template <class T, bool KAHAN>
 struct Acc;
@@ -117,12 +118,12 @@ __host__ __device__ constexpr int tap_chunks(int p12) { return (p12 - 1 + kCH - 
 template <class T, int P0, bool KAHAN>
 __global__ __launch_bounds__(kThreads) void ar_wave_kernel(
 	T* __restrict__ zeta, int nt, int nx, int ny, const T* __restrict__ phi0,
-	const T* __restrict__ phic, int P1, int P2, int nY, int* __restrict__ progress,
-	const T* __restrict__ zero, T* __restrict__ sink, int total_steps) {
+	const T* __restrict__ phic, const int* __restrict__ toff, int P1, int P2, int nY,
+	int* __restrict__ progress, const T* __restrict__ zero, T* __restrict__ sink,
+	int total_steps) {
 	extern __shared__ __align__(16) unsigned char smem_raw[];
 	const int P12 = P1 * P2;
-	T* lzero = reinterpret_cast<T*>(smem_raw);  // 16 zero bytes
-	T* ring = lzero + 16 / sizeof(T);
+	T* ring = reinterpret_cast<T*>(smem_raw);
 	const int HX = P1 - 1, HY = P2 - 1;
 	const int WX = kTX + HX, WY = kTY + HY;
 	const int R = P1 + P2 - 1;
@@ -172,6 +173,9 @@ __global__ __launch_bounds__(kThreads) void ar_wave_kernel(
 	// observed it far enough ahead before publishing the progress we wait for.
 	const int nb_top = X > 0 ? b - nY : -1;
 	const int nb_left = Y > 0 ? b - 1 : -1;
+	int* const my_prog = progress + size_t(b) * kProgStride;
+	const int* const top_prog = progress + size_t(max(nb_top, 0)) * kProgStride;
+	const int* const left_prog = progress + size_t(max(nb_left, 0)) * kProgStride;
 
 	Acc<T, KAHAN> acc[P0];
 #pragma unroll
@@ -191,7 +195,6 @@ __global__ __launch_bounds__(kThreads) void ar_wave_kernel(
 
 	// the ring starts zeroed: halo positions outside the grid stay zero
 	for (int q = tid; q < R * slot; q += kThreads) ring[q] = T(0);
-	if (tid < 16 / int(sizeof(T))) lzero[tid] = T(0);
 	// ring slots (slice t lives in slot t mod R), advanced every step
 	int rs_own = pmod(s0 - lx - ly, R);
 	int rs_h0 = pmod(s0 - hsk[0], R);
@@ -220,8 +223,8 @@ __global__ __launch_bounds__(kThreads) void ar_wave_kernel(
 	// flight (read kD steps after they are issued)
 	int known_t = total_u, known_l = total_u;
 	if (tid == 0) {
-		known_t = nb_top >= 0 ? wait_progress(progress + nb_top, need_top(kD - 1)) : total_u;
-		known_l = nb_left >= 0 ? wait_progress(progress + nb_left, need_left(kD - 1)) : total_u;
+		known_t = nb_top >= 0 ? wait_progress(top_prog, need_top(kD - 1)) : total_u;
+		known_l = nb_left >= 0 ? wait_progress(left_prog, need_left(kD - 1)) : total_u;
 	}
 	__syncthreads();
 	T qe[kD], qh0[kD], qh1[kD];
@@ -251,20 +254,16 @@ __global__ __launch_bounds__(kThreads) void ar_wave_kernel(
 			T part[P0];
 #pragma unroll
 			for (int k = 0; k < P0; ++k) part[k] = T(0);
-			// taps m = 1 .. P12-1 in chunks of kCH; taps past the last one
-			// have zero coefficients and read a zero
-			int ci = 0, cj = 0;  // (i, j) of the last tap read
+			// taps m = 1 .. P12-1 in chunks of kCH; toff holds each tap's
+			// LDS byte distance back from the own position (taps past the
+			// last one have zero coefficients and read the own position)
 			const T* pc = phic;
-			for (int c0 = 1; c0 < P12; c0 += kCH, pc += P0 * kCH) {
+			const int* oc = toff;
+			const char* bb = reinterpret_cast<const char*>(base);
+			for (int c0 = 1; c0 < P12; c0 += kCH, pc += P0 * kCH, oc += kCH) {
 				T v[kCH];
 #pragma unroll
-				for (int q = 0; q < kCH; ++q) {
-					if (++cj == P2) {
-						cj = 0;
-						++ci;
-					}
-					v[q] = *(c0 + q < P12 ? base - (ci * WY + cj) : lzero);
-				}
+				for (int q = 0; q < kCH; ++q) v[q] = *reinterpret_cast<const T*>(bb - oc[q]);
 #pragma unroll
 				for (int k = 0; k < P0; ++k) {
 #pragma unroll
@@ -291,18 +290,16 @@ __global__ __launch_bounds__(kThreads) void ar_wave_kernel(
 			known_t = max(known_t, t_q);
 			known_l = max(known_l, l_q);
 			if (u2 < total_u) {
-				if (known_t < need_top(u2))
-					known_t = wait_progress(progress + nb_top, need_top(u2));
-				if (known_l < need_left(u2))
-					known_l = wait_progress(progress + nb_left, need_left(u2));
+				if (known_t < need_top(u2)) known_t = wait_progress(top_prog, need_top(u2));
+				if (known_l < need_left(u2)) known_l = wait_progress(left_prog, need_left(u2));
 			}
-			t_q = nb_top >= 0 ? coherent_load(progress + nb_top) : total_u;
-			l_q = nb_left >= 0 ? coherent_load(progress + nb_left) : total_u;
+			t_q = nb_top >= 0 ? coherent_load(top_prog) : total_u;
+			l_q = nb_left >= 0 ? coherent_load(left_prog) : total_u;
 		}
 		__syncthreads();
 		// 4. publish: the stores of step u - kD were issued before the queue
 		//    loads consumed at this step, so every wave has seen them complete
-		if (tid == 0 && u >= kD) coherent_store(progress + b, u - kD + 1);
+		if (tid == 0 && u >= kD) coherent_store(my_prog, u - kD + 1);
 		// 5. loads for step u + kD into the queue slot just consumed
 		issue_loads(u2, e_q, h0_q, h1_q);
 	};
@@ -324,7 +321,7 @@ __global__ __launch_bounds__(kThreads) void ar_wave_kernel(
 	// every store complete, then the final count
 	__builtin_amdgcn_s_waitcnt(0);
 	__syncthreads();
-	if (tid == 0) coherent_store(progress + b, total_u);
+	if (tid == 0) coherent_store(my_prog, total_u);
 }
 
 template <class T, bool KAHAN>
@@ -347,14 +344,22 @@ void launch_typed(arma_hip_ctx* ctx, const arma_ar_desc* d, T* zeta) {
 			ph[n0 + (size_t(ch) * P0 + k) * kCH + q] = T(pk[m]);
 		}
 	}
-	T* dphi = (T*)ctx->phi.get(ph.size() * sizeof(T));
+	// tap offsets in LDS bytes, back from the own position: (i*WY + j) slots
+	const int WY = kTY + P2 - 1;
+	std::vector<int> offs(size_t(nch) * kCH, 0);
+	for (int m = 1; m < P12; ++m) offs[m - 1] = ((m / P2) * WY + (m % P2)) * int(sizeof(T));
+	const size_t phi_bytes = (ph.size() * sizeof(T) + 255) / 256 * 256;
+	unsigned char* dbuf = (unsigned char*)ctx->phi.get(phi_bytes + offs.size() * sizeof(int) + 4);
+	T* dphi = (T*)dbuf;
+	int* doffs = (int*)(dbuf + phi_bytes);
 	upload(ctx, dphi, ph.data(), ph.size() * sizeof(T));
+	if (!offs.empty()) upload(ctx, doffs, offs.data(), offs.size() * sizeof(int));
 	const int nX = div_ceil(nx, kTX), nY = div_ceil(ny, kTY);
 	const int ntiles = nX * nY;
 	const int total_steps = nt + kTX + kTY - 2;
 	// flags buffer: progress counters, a zero word for out-of-range loads,
 	// and a per-tile sink for out-of-range stores
-	const size_t prog_bytes = (sizeof(int) * size_t(ntiles) + 255) / 256 * 256;
+	const size_t prog_bytes = sizeof(int) * kProgStride * size_t(ntiles);
 	const size_t zero_bytes = 256;
 	unsigned char* fb = (unsigned char*)ctx->flags.get(prog_bytes + zero_bytes +
 	                                                   size_t(ntiles) * kThreads * sizeof(T));
@@ -365,7 +370,7 @@ void launch_typed(arma_hip_ctx* ctx, const arma_ar_desc* d, T* zeta) {
 	reset_progress<<<div_ceil(nclear, 256), 256, 0, ctx->stream>>>(prog, nclear);
 	ARMA_HIP_CHECK(hipGetLastError());
 	const int R = P1 + P2 - 1;
-	const size_t lds = 16 + size_t(R) * (kTX + P1 - 1) * (kTY + P2 - 1) * sizeof(T);
+	const size_t lds = size_t(R) * (kTX + P1 - 1) * (kTY + P2 - 1) * sizeof(T);
 	ARMA_HIP_REQUIRE(lds <= 160 * 1024, ARMA_HIP_EINVAL, "AR order too large for the LDS ring");
 	KernelTimer timer(ctx, "ar_wavefront");
 	// rings above 64 KiB (P1, P2 >= 9 in FP64) use the larger LDS of gfx950
@@ -376,7 +381,8 @@ void launch_typed(arma_hip_ctx* ctx, const arma_ar_desc* d, T* zeta) {
 				reinterpret_cast<const void*>(&ar_wave_kernel<T, P, KAHAN>),                  \
 				hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));                       \
 		ar_wave_kernel<T, P, KAHAN><<<ntiles, kThreads, lds, ctx->stream>>>(                  \
-			zeta, nt, nx, ny, dphi, dphi + n0, P1, P2, nY, prog, zero, sink, total_steps);    \
+			zeta, nt, nx, ny, dphi, dphi + n0, doffs, P1, P2, nY, prog, zero, sink,           \
+			total_steps);                                                                     \
 		break;
 	switch (P0) {
 		ARMA_WAVE(1)

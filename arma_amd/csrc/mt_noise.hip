@@ -393,7 +393,6 @@ __global__ __launch_bounds__(64) void noise_seg_kernel(
 	constexpr int RW = kSegLanes * WPA;                     // words per round
 	__shared__ uint32_t st_all[2][32];
 	__shared__ uint32_t buf_all[2][RW + 32];
-	__shared__ int more[2];
 	const int seg = threadIdx.x >> 5;
 	const int lane = threadIdx.x & 31;
 	const int pid = blockIdx.x * 2 + seg;
@@ -432,21 +431,13 @@ __global__ __launch_bounds__(64) void noise_seg_kernel(
 	__syncthreads();
 	int avail = 0;
 	long long produced = 0;
-	for (;;) {
-		if (lane == 0) more[seg] = produced < n;
-		__syncthreads();
-		const bool any = more[0] || more[1];
-		__syncthreads();
-		if (!any) break;
+	// the workgroup is one wave: loop conditions are wave votes, and the
+	// barriers below only order LDS accesses between lanes
+	while (__any(produced < n)) {
 		// twists until a round of words is buffered (avail is the same on
 		// every lane of a segment)
-		while (true) {
-			if (lane == 0) more[seg] = avail < RW;
-			__syncthreads();
-			const bool need = more[0] || more[1];
-			const bool mine = more[seg];
-			__syncthreads();
-			if (!need) break;
+		while (__any(avail < RW)) {
+			const bool mine = avail < RW;
 			uint32_t w = 0;
 			if (mine && lane < nn) {
 				const int k = lane, m = p.mm;
