@@ -102,7 +102,7 @@ void arma_hip_destroy(arma_hip_ctx* ctx) {
 	(void)hipSetDevice(ctx->device);
 	(void)hipStreamSynchronize(ctx->stream);
 	for (arma_hip::Scratch* s : {&ctx->params, &ctx->seeds, &ctx->state, &ctx->phi,
-	                             &ctx->partials, &ctx->flags, &ctx->noise_tmp, &ctx->pot_spec,
+	                             &ctx->partials, &ctx->flags, &ctx->noise_tmp, &ctx->mbox, &ctx->pot_spec,
 	                             &ctx->pot_mid, &ctx->pot_wn, &ctx->pot_twiddle, &ctx->pot_kmin,
 	                             &ctx->zeta_tmp, &ctx->phi_tmp})
 		s->release();

@@ -16,15 +16,21 @@
 //  * LDS holds a ring of R = P1+P2-1 slices of the tile plus its causal halo
 //    (P1-1 rows above, P2-1 columns to the left): slot t mod R of position p
 //    is rewritten only after every reader of slice t is done.
-//  * Halo values come from the neighbouring tiles through HBM.  Surface
-//    stores and halo loads are device-coherent (agent-scope relaxed atomics:
-//    they write through, and read past, the per-XCD L2), so publishing needs
-//    no cache-wide write-back or invalidate: every wave waits for its stores
-//    to complete, the workgroup meets at a barrier, and one lane stores the
-//    number of steps completed.  A consumer polls its neighbours' counters
-//    (relaxed loads, with an acquire load every 64 polls) before issuing the
-//    halo loads, which are prefetched D steps ahead.  Tiles only wait on
-//    tiles with smaller workgroup ids, so the launch cannot deadlock.
+//  * Halo values come from the neighbouring tiles through HBM.  Each tile
+//    keeps a mailbox with one row per step of the values its boundary lanes
+//    (the bottom P1-1 rows and right P2-1 columns) computed at that step,
+//    stored device-coherently (agent-scope relaxed atomics, which write
+//    through the per-XCD L2) as contiguous rows; the surface itself goes out
+//    with ordinary write-back stores.  A neighbour computed the value at a
+//    halo position at its own step u + 16 (edge) or u + 32 (corner), so a
+//    consumer reads a fixed mailbox row per step.  A tile publishes the
+//    number of steps whose mailbox rows are complete (each wave's stores are
+//    older than the queue loads it has consumed, and one lane stores the
+//    count after a barrier), and polls its neighbours' counters (relaxed
+//    device-coherent loads, an acquire load every 64 polls) before issuing
+//    the halo loads, which are prefetched D steps ahead.  Counters sit on
+//    separate cache lines.  Tiles only wait on tiles with smaller workgroup
+//    ids, so the launch cannot deadlock.
 //  * In-slice taps go in chunks of 16: the chunk's ring values are read from
 //    LDS into registers once, and the coefficients (uniform across the
 //    workgroup) arrive through the scalar cache in rows of 16 and enter the
@@ -115,12 +121,25 @@ constexpr int kCH = 16;
 
 __host__ __device__ constexpr int tap_chunks(int p12) { return (p12 - 1 + kCH - 1) / kCH; }
 
+// Mailbox column of tile-local lane (x, y), or -1 when no neighbour reads it:
+// the bottom HX rows first (row-major), then the right HY columns of the
+// rows above them.
+__host__ __device__ __forceinline__ int mbox_column(int x, int y, int HX, int HY) {
+	if (x >= kTX - HX) return (x - (kTX - HX)) * kTY + y;
+	if (y >= kTY - HY) return kTY * HX + x * HY + (y - (kTY - HY));
+	return -1;
+}
+
+__host__ __device__ constexpr int mbox_width(int HX, int HY) {
+	return kTY * HX + (kTX - HX) * HY;
+}
+
 template <class T, int P0, bool KAHAN>
 __global__ __launch_bounds__(kThreads) void ar_wave_kernel(
 	T* __restrict__ zeta, int nt, int nx, int ny, const T* __restrict__ phi0,
 	const T* __restrict__ phic, const int* __restrict__ toff, int P1, int P2, int nY,
 	int* __restrict__ progress, const T* __restrict__ zero, T* __restrict__ sink,
-	int total_steps) {
+	T* __restrict__ mbox, int NB, int total_steps) {
 	extern __shared__ __align__(16) unsigned char smem_raw[];
 	const int P12 = P1 * P2;
 	T* ring = reinterpret_cast<T*>(smem_raw);
@@ -141,7 +160,7 @@ __global__ __launch_bounds__(kThreads) void ar_wave_kernel(
 	// halo positions: top band (hx < 0, all hy), then left band (hx >= 0, hy < 0)
 	const int H = WX * WY - kTX * kTY;
 	int hsk[2], hlds[2];  // skew hx + hy and LDS position of each halo slot
-	size_t hoff[2];
+	int hnb[2], hm[2], hdu[2];  // producing tile, its mailbox column, step offset
 	bool hval[2], hin[2];
 #pragma unroll
 	for (int r = 0; r < 2; ++r) {
@@ -151,7 +170,9 @@ __global__ __launch_bounds__(kThreads) void ar_wave_kernel(
 		hval[r] = false;
 		hsk[r] = 0;
 		hlds[r] = 0;
-		hoff[r] = 0;
+		hnb[r] = 0;
+		hm[r] = 0;
+		hdu[r] = 0;
 		if (h < H) {
 			if (h < HX * WY) {
 				hx = -HX + h / WY;
@@ -166,7 +187,12 @@ __global__ __launch_bounds__(kThreads) void ar_wave_kernel(
 			hlds[r] = (hx + HX) * WY + (hy + HY);
 			const int gx = X0 + hx, gy = Y0 + hy;
 			hin[r] = gx >= 0 && gy >= 0 && gx < nx && gy < ny;
-			if (hin[r]) hoff[r] = size_t(gx) * ny + gy;
+			// the producing tile computed this position at its lane (px, py),
+			// at u-step u + 16 (edge neighbours) or u + 32 (corner)
+			const int px = hx < 0 ? kTX + hx : hx, py = hy < 0 ? kTY + hy : hy;
+			hnb[r] = b - (hx < 0 ? nY : 0) - (hy < 0 ? 1 : 0);
+			hm[r] = mbox_column(px, py, HX, HY);
+			hdu[r] = (hx < 0 ? kTX : 0) + (hy < 0 ? kTY : 0);
 		}
 	}
 	// Neighbour tiles.  The corner tile needs no poll: the top tile had
@@ -205,6 +231,14 @@ __global__ __launch_bounds__(kThreads) void ar_wave_kernel(
 	// scratch sink), so the vector memory counter waits for exactly the queue
 	// slot being consumed and no load result needs a select at issue.
 	T* const my_sink = sink + size_t(b) * kThreads + tid;
+	// Boundary lanes (the bottom HX rows and the right HY columns, which the
+	// tiles below and to the right read as halo) also store every step's
+	// value into this tile's mailbox, row u, column my_m: whole rows per
+	// step, so the device-coherent stores are contiguous.  The surface
+	// itself is stored with ordinary (write-back) stores.
+	const int my_m = mbox_column(lx, ly, HX, HY);
+	T* const my_mbox = my_m >= 0 ? mbox + size_t(b) * total_u * NB + my_m : my_sink;
+	const size_t mstep = my_m >= 0 ? size_t(NB) : 0;
 	auto issue_loads = [&](int u2, T& e, T& h0, T& h1) {
 		const int s2 = u2 + s0;
 		const bool in = u2 < total_u;
@@ -215,8 +249,10 @@ __global__ __launch_bounds__(kThreads) void ar_wave_kernel(
 		const int t1 = s2 - hsk[1];
 		const bool v1 = in && hin[1] && t1 >= 0 && t1 < nt;
 		e = *(ve ? zeta + size_t(t) * sxy + own_off : zero);
-		h0 = coherent_load(v0 ? zeta + size_t(t0) * sxy + hoff[0] : zero);
-		h1 = coherent_load(v1 ? zeta + size_t(t1) * sxy + hoff[1] : zero);
+		h0 = coherent_load(v0 ? mbox + (size_t(hnb[0]) * total_u + u2 + hdu[0]) * NB + hm[0]
+		                      : zero);
+		h1 = coherent_load(v1 ? mbox + (size_t(hnb[1]) * total_u + u2 + hdu[1]) * NB + hm[1]
+		                      : zero);
 	};
 
 	// neighbour progress as last observed by lane 0, and the flag loads in
@@ -265,16 +301,17 @@ __global__ __launch_bounds__(kThreads) void ar_wave_kernel(
 #pragma unroll
 				for (int q = 0; q < kCH; ++q) v[q] = *reinterpret_cast<const T*>(bb - oc[q]);
 #pragma unroll
-				for (int k = 0; k < P0; ++k) {
+				for (int q = 0; q < kCH; ++q) {
 #pragma unroll
-					for (int q = 0; q < kCH; ++q) part[k] = fma(pc[k * kCH + q], v[q], part[k]);
+					for (int k = 0; k < P0; ++k) part[k] = fma(pc[q * P0 + k], v[q], part[k]);
 				}
 			}
 			acc[0].add(part[0]);
 			const T eps = e_q;
 			const T z = eps + (acc[0].value() + phi000 * eps);
 			if (live) ring[rs_own * slot + own_lds] = z;
-			coherent_store(live ? zeta + size_t(t) * sxy + own_off : my_sink, z);
+			*(live ? zeta + size_t(t) * sxy + own_off : my_sink) = z;
+			coherent_store(my_mbox + size_t(u) * mstep, z);
 #pragma unroll
 			for (int k = 1; k < P0; ++k) acc[k].add(fma(phi0[k], z, part[k]));
 #pragma unroll
@@ -341,7 +378,7 @@ void launch_typed(arma_hip_ctx* ctx, const arma_ar_desc* d, T* zeta) {
 		ph[k] = T(pk[0]);
 		for (int m = 1; m < P12; ++m) {
 			const int ch = (m - 1) / kCH, q = (m - 1) % kCH;
-			ph[n0 + (size_t(ch) * P0 + k) * kCH + q] = T(pk[m]);
+			ph[n0 + (size_t(ch) * kCH + q) * P0 + k] = T(pk[m]);
 		}
 	}
 	// tap offsets in LDS bytes, back from the own position: (i*WY + j) slots
@@ -366,6 +403,10 @@ void launch_typed(arma_hip_ctx* ctx, const arma_ar_desc* d, T* zeta) {
 	int* prog = (int*)fb;
 	const T* zero = (const T*)(fb + prog_bytes);
 	T* sink = (T*)(fb + prog_bytes + zero_bytes);
+	// mailboxes: one row of boundary values per tile and step
+	const int NB = std::max(1, mbox_width(P1 - 1, P2 - 1));
+	const int total_u = total_steps + (P1 - 1) + (P2 - 1);
+	T* mbox = (T*)ctx->mbox.get(sizeof(T) * size_t(ntiles) * total_u * NB);
 	const int nclear = int((prog_bytes + zero_bytes) / sizeof(int));
 	reset_progress<<<div_ceil(nclear, 256), 256, 0, ctx->stream>>>(prog, nclear);
 	ARMA_HIP_CHECK(hipGetLastError());
@@ -381,7 +422,7 @@ void launch_typed(arma_hip_ctx* ctx, const arma_ar_desc* d, T* zeta) {
 				reinterpret_cast<const void*>(&ar_wave_kernel<T, P, KAHAN>),                  \
 				hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));                       \
 		ar_wave_kernel<T, P, KAHAN><<<ntiles, kThreads, lds, ctx->stream>>>(                  \
-			zeta, nt, nx, ny, dphi, dphi + n0, doffs, P1, P2, nY, prog, zero, sink,           \
+			zeta, nt, nx, ny, dphi, dphi + n0, doffs, P1, P2, nY, prog, zero, sink, mbox, NB, \
 			total_steps);                                                                     \
 		break;
 	switch (P0) {

@@ -83,7 +83,7 @@ struct arma_hip_ctx {
 	bool own_stream = false;
 	int num_cus = 0;
 	// named scratch areas
-	arma_hip::Scratch params, seeds, state, phi, partials, flags, noise_tmp;
+	arma_hip::Scratch params, seeds, state, phi, partials, flags, noise_tmp, mbox;
 	arma_hip::Scratch pot_spec, pot_mid, pot_wn, pot_twiddle, pot_kmin, zeta_tmp, phi_tmp;
 	int twiddle_n = 0;
 };
