@@ -134,20 +134,20 @@ __host__ __device__ constexpr int mbox_width(int HX, int HY) {
 	return kTY * HX + (kTX - HX) * HY;
 }
 
+// The march of one tile over all of t (P0 accumulators; a surface of lower
+// order in t has zero coefficients past its own order).
 template <class T, int P0, bool KAHAN>
-__global__ __launch_bounds__(kThreads) void ar_wave_kernel(
+__device__ __forceinline__ void wave_tile(
 	T* __restrict__ zeta, int nt, int nx, int ny, const T* __restrict__ phi0,
 	const T* __restrict__ phic, const int* __restrict__ toff, int P1, int P2, int nY,
 	int* __restrict__ progress, const T* __restrict__ zero, T* __restrict__ sink,
-	T* __restrict__ mbox, int NB, int total_steps) {
-	extern __shared__ __align__(16) unsigned char smem_raw[];
+	T* __restrict__ mbox, int NB, int total_steps, int b, unsigned char* smem_raw) {
 	const int P12 = P1 * P2;
 	T* ring = reinterpret_cast<T*>(smem_raw);
 	const int HX = P1 - 1, HY = P2 - 1;
 	const int WX = kTX + HX, WY = kTY + HY;
 	const int R = P1 + P2 - 1;
 	const int slot = WX * WY;
-	const int b = blockIdx.x;
 	const int X = b / nY, Y = b - (b / nY) * nY;
 	const int X0 = X * kTX, Y0 = Y * kTY;
 	const int tid = threadIdx.x;
@@ -359,6 +359,20 @@ __global__ __launch_bounds__(kThreads) void ar_wave_kernel(
 	__builtin_amdgcn_s_waitcnt(0);
 	__syncthreads();
 	if (tid == 0) coherent_store(my_prog, total_u);
+}
+
+// Kernel arguments stay plain restrict-qualified pointers: the coefficient
+// reads are then provably unclobbered by the kernel's stores and compile to
+// scalar loads (a struct argument costs 30-60 VGPRs per lane).
+template <class T, int P0, bool KAHAN>
+__global__ __launch_bounds__(kThreads) void ar_wave_kernel(
+	T* __restrict__ zeta, int nt, int nx, int ny, const T* __restrict__ phi0,
+	const T* __restrict__ phic, const int* __restrict__ toff, int P1, int P2, int nY,
+	int* __restrict__ progress, const T* __restrict__ zero, T* __restrict__ sink,
+	T* __restrict__ mbox, int NB, int total_steps) {
+	extern __shared__ __align__(16) unsigned char smem_raw[];
+	wave_tile<T, P0, KAHAN>(zeta, nt, nx, ny, phi0, phic, toff, P1, P2, nY, progress, zero, sink,
+	                        mbox, NB, total_steps, blockIdx.x, smem_raw);
 }
 
 template <class T, bool KAHAN>
