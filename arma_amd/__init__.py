@@ -6,7 +6,9 @@ The C++ drop-in for the reference's generator and solver registry lives in
 arma_amd/host/; this Python package is a thin ctypes binding used by tests
 and benchmarks.
 """
-from ._abi import (AR_PLAIN, AR_WAVEFRONT, FP32, FP64, NOISE_DOUBLE, NOISE_FLOAT,  # noqa: F401
-                   NOISE_PARTITIONED, NOISE_SEQUENTIAL, ArmaHipError, Context, MtParams,
+from ._abi import (AR_PLAIN, AR_WAVEFRONT, BATCH_AUTO, BATCH_INTERLEAVED,  # noqa: F401
+                   BATCH_SURFACE_MAJOR, FP32, FP64, NOISE_DOUBLE, NOISE_FLOAT,
+                   NOISE_PARTITIONED, NOISE_SEQUENTIAL, ArmaHipError, Batch, Context, MtParams,
+                   Server,
                    PRNGError, dcmt_create, dcmt_create_many, lib, mt19937_params,
                    read_mt_config, write_mt_config)

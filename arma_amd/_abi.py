@@ -25,6 +25,8 @@ FP64, FP32 = 0, 1
 NOISE_SEQUENTIAL, NOISE_PARTITIONED = 0, 1
 NOISE_DOUBLE, NOISE_FLOAT = 0, 1
 AR_PLAIN, AR_WAVEFRONT = 0, 1
+BATCH_AUTO, BATCH_SURFACE_MAJOR, BATCH_INTERLEAVED = 0, 1, 2
+REQ_SURFACE, REQ_STREAM_CHUNK = 0, 1
 
 EXPORTS = [
     "arma_hip_init", "arma_hip_destroy", "arma_hip_last_error", "arma_hip_set_stream",
@@ -34,7 +36,13 @@ EXPORTS = [
     "arma_hip_write_mt_config", "arma_hip_dcmt_create", "arma_hip_dcmt_create_many",
     "arma_hip_acf", "arma_hip_yule_walker_choi", "arma_hip_yule_walker_gauss",
     "arma_hip_white_noise_variance", "arma_hip_partition_shape", "arma_hip_profile",
-    "arma_hip_profile_report",
+    "arma_hip_profile_report", "arma_hip_batch_create", "arma_hip_batch_destroy",
+    "arma_hip_batch_layout", "arma_hip_batch_generate", "arma_hip_batch_capture",
+    "arma_hip_batch_replay", "arma_hip_batch_tune", "arma_hip_batch_get_config",
+    "arma_hip_batch_set_config", "arma_hip_batch_factors", "arma_hip_server_clock",
+    "arma_hip_server_create", "arma_hip_server_destroy", "arma_hip_server_open_stream",
+    "arma_hip_server_close_stream", "arma_hip_server_submit", "arma_hip_server_poll",
+    "arma_hip_server_release", "arma_hip_server_stats", "arma_hip_server_copy",
 ]
 
 
@@ -67,6 +75,49 @@ class PotDesc(C.Structure):
         ("lbound", C.c_double * 2), ("ubound", C.c_double * 2), ("npoints", C.c_int32 * 2),
         ("precision", C.c_int32),
     ]
+
+
+class BatchConfig(C.Structure):
+    _fields_ = [("tile_x", C.c_int32), ("schedule", C.c_int32), ("slots_per_cu", C.c_int32)]
+
+    def as_tuple(self):
+        return (self.tile_x, self.schedule, self.slots_per_cu)
+
+
+class BatchDesc(C.Structure):
+    _fields_ = [("n_surfaces", C.c_int32), ("surfaces", C.POINTER(ArDesc)),
+                ("gamma0", C.POINTER(C.c_double)), ("history", C.POINTER(C.c_int32)),
+                ("precision", C.c_int32),
+                ("config", BatchConfig)]
+
+
+class BatchTrial(C.Structure):
+    _fields_ = [("config", BatchConfig), ("grid", C.c_int32), ("ms", C.c_double)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("lanes", C.c_int32), ("max_batch", C.c_int32), ("hbm_budget", C.c_int64),
+                ("precision", C.c_int32), ("batch", BatchConfig)]
+
+
+class Request(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("surface", ArDesc), ("gamma0", C.c_double),
+                ("want_potential", C.c_int32), ("potential", PotDesc), ("stream", C.c_int64),
+                ("arrival", C.c_double)]
+
+
+class Completion(C.Structure):
+    _fields_ = [("id", C.c_int64), ("status", C.c_int32), ("kind", C.c_int32),
+                ("stream", C.c_int64), ("chunk", C.c_int64), ("zeta", C.c_void_p),
+                ("phi", C.c_void_p), ("shape", C.c_int32 * 3), ("t_arrival", C.c_double),
+                ("t_admit", C.c_double), ("t_done", C.c_double), ("batch_size", C.c_int32),
+                ("lane", C.c_int32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("in_use", C.c_int64), ("peak", C.c_int64), ("budget", C.c_int64),
+                ("batches", C.c_int64), ("batched_requests", C.c_int64),
+                ("completed", C.c_int64), ("pending", C.c_int64), ("busy_lanes", C.c_int32)]
 
 
 def mt19937_params():
@@ -129,6 +180,33 @@ def lib():
         L.arma_hip_white_noise_variance.restype = C.c_double
         L.arma_hip_partition_shape.argtypes = [i3, i3, i3, C.c_int32, i3]
         L.arma_hip_partition_shape.restype = None
+        L.arma_hip_batch_create.argtypes = [vp, C.POINTER(BatchDesc), C.POINTER(vp)]
+        L.arma_hip_batch_destroy.argtypes = [vp]
+        L.arma_hip_batch_destroy.restype = None
+        L.arma_hip_batch_layout.argtypes = [vp, C.POINTER(C.c_int64)]
+        L.arma_hip_batch_generate.argtypes = [vp, vp, vp]
+        L.arma_hip_batch_capture.argtypes = [vp, vp, vp]
+        L.arma_hip_batch_replay.argtypes = [vp, vp]
+        L.arma_hip_batch_tune.argtypes = [vp, vp, vp, C.c_int32, C.POINTER(BatchTrial),
+                                          C.c_int32, C.POINTER(C.c_int32)]
+        L.arma_hip_batch_get_config.argtypes = [vp, C.POINTER(BatchConfig),
+                                                C.POINTER(C.c_int32)]
+        L.arma_hip_batch_set_config.argtypes = [vp, vp, C.POINTER(BatchConfig)]
+        L.arma_hip_batch_factors.argtypes = [vp, vp, C.POINTER(C.c_double)]
+        L.arma_hip_server_clock.argtypes = []
+        L.arma_hip_server_clock.restype = C.c_double
+        L.arma_hip_server_create.argtypes = [C.c_int, C.POINTER(ServerConfig), C.POINTER(vp)]
+        L.arma_hip_server_destroy.argtypes = [vp]
+        L.arma_hip_server_destroy.restype = None
+        L.arma_hip_server_open_stream.argtypes = [vp, C.POINTER(ArDesc), C.c_int32,
+                                                  C.POINTER(C.c_int64)]
+        L.arma_hip_server_close_stream.argtypes = [vp, C.c_int64]
+        L.arma_hip_server_submit.argtypes = [vp, C.POINTER(Request), C.POINTER(C.c_int64)]
+        L.arma_hip_server_poll.argtypes = [vp, C.POINTER(Completion), C.c_int32,
+                                           C.POINTER(C.c_int32), C.c_int32]
+        L.arma_hip_server_release.argtypes = [vp, C.c_int64]
+        L.arma_hip_server_stats.argtypes = [vp, C.POINTER(ServerStats)]
+        L.arma_hip_server_copy.argtypes = [vp, vp, vp, C.c_size_t]
         L.arma_hip_profile.argtypes = [vp, C.c_int32]
         L.arma_hip_profile_report.argtypes = [vp, C.c_char_p, C.c_size_t, C.c_int32]
         _lib = L
@@ -354,3 +432,166 @@ class Context:
         check(lib().arma_hip_mt_raw(self.h, C.cast(marr, C.POINTER(MtParams)),
                                     None if sarr is None else sarr.ctypes.data, len(marr), draws,
                                     C.c_void_p(_ptr(out))))
+
+    def batch(self, descs, gamma0=None, precision=FP32, config=(0, 0, 0), history=None):
+        return Batch(self, descs, gamma0, precision, config, history)
+
+
+class Batch:
+    """Many surfaces generated by one persistent recurrence launch
+    (arma_hip_batch_*).  descs are ar_desc() descriptors; surface i lives at
+    arena[offsets[i] : offsets[i] + prod(shape_i)]."""
+
+    def __init__(self, ctx, descs, gamma0=None, precision=FP32, config=(0, 0, 0),
+                 history=None):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        self._descs = list(descs)
+        arr = (ArDesc * len(self._descs))(*self._descs)
+        g = None
+        if gamma0 is not None:
+            g = np.ascontiguousarray(gamma0, np.float64)
+        bd = BatchDesc()
+        bd.n_surfaces = len(self._descs)
+        bd.surfaces = C.cast(arr, C.POINTER(ArDesc))
+        bd.gamma0 = None if g is None else g.ctypes.data_as(C.POINTER(C.c_double))
+        h = None
+        if history is not None:
+            h = np.ascontiguousarray(history, np.int32)
+            bd.history = h.ctypes.data_as(C.POINTER(C.c_int32))
+        bd.precision = precision
+        bd.config = BatchConfig(*config)
+        check(lib().arma_hip_batch_create(ctx.h, C.byref(bd), C.byref(self.h)))
+        self.precision = precision
+        self.shapes = [tuple(d.shape) for d in self._descs]
+        offs = (C.c_int64 * (len(self._descs) + 1))()
+        check(lib().arma_hip_batch_layout(self.h, offs))
+        self.offsets = [int(v) for v in offs]
+
+    @property
+    def arena_elements(self):
+        return self.offsets[-1]
+
+    def surface(self, arena, i):
+        n = int(np.prod(self.shapes[i]))
+        return arena[self.offsets[i]:self.offsets[i] + n].view(self.shapes[i])
+
+    def generate(self, arena):
+        check(lib().arma_hip_batch_generate(self.ctx.h, self.h, C.c_void_p(_ptr(arena))))
+
+    def capture(self, arena):
+        check(lib().arma_hip_batch_capture(self.ctx.h, self.h, C.c_void_p(_ptr(arena))))
+
+    def replay(self):
+        check(lib().arma_hip_batch_replay(self.ctx.h, self.h))
+
+    def tune(self, arena, reps=2):
+        trials = (BatchTrial * 64)()
+        n = C.c_int32()
+        check(lib().arma_hip_batch_tune(self.ctx.h, self.h, C.c_void_p(_ptr(arena)), reps,
+                                        trials, 64, C.byref(n)))
+        return [(trials[i].config.as_tuple(), trials[i].grid, trials[i].ms)
+                for i in range(n.value)]
+
+    def config(self):
+        c = BatchConfig()
+        g = C.c_int32()
+        check(lib().arma_hip_batch_get_config(self.h, C.byref(c), C.byref(g)))
+        return c.as_tuple(), g.value
+
+    def set_config(self, config):
+        c = BatchConfig(*config)
+        check(lib().arma_hip_batch_set_config(self.ctx.h, self.h, C.byref(c)))
+
+    def factors(self):
+        out = np.zeros(len(self._descs), np.float64)
+        check(lib().arma_hip_batch_factors(self.ctx.h, self.h,
+                                           out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def close(self):
+        if self.h:
+            lib().arma_hip_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Request loop with continuous batching (arma_hip_server_*)."""
+
+    def __init__(self, device=0, lanes=2, budget=1 << 30, precision=FP32, max_batch=0,
+                 config=(0, 0, 0)):
+        self.h = C.c_void_p()
+        cfg = ServerConfig(lanes, max_batch, int(budget), precision, BatchConfig(*config))
+        check(lib().arma_hip_server_create(device, C.byref(cfg), C.byref(self.h)))
+        self.precision = precision
+
+    @staticmethod
+    def clock():
+        return lib().arma_hip_server_clock()
+
+    def open_stream(self, desc, chunk):
+        sid = C.c_int64()
+        check(lib().arma_hip_server_open_stream(self.h, C.byref(desc), chunk, C.byref(sid)))
+        return sid.value
+
+    def close_stream(self, sid):
+        check(lib().arma_hip_server_close_stream(self.h, sid))
+
+    def submit_surface(self, desc, gamma0=0.0, pot=None, arrival=0.0):
+        r = Request()
+        r.kind = REQ_SURFACE
+        r.surface = desc
+        r.gamma0 = gamma0
+        if pot is not None:
+            r.want_potential = 1
+            r.potential = pot
+        r.arrival = arrival
+        rid = C.c_int64()
+        check(lib().arma_hip_server_submit(self.h, C.byref(r), C.byref(rid)))
+        return rid.value
+
+    def submit_chunk(self, sid, arrival=0.0):
+        r = Request()
+        r.kind = REQ_STREAM_CHUNK
+        r.stream = sid
+        r.arrival = arrival
+        rid = C.c_int64()
+        check(lib().arma_hip_server_submit(self.h, C.byref(r), C.byref(rid)))
+        return rid.value
+
+    def poll(self, max_n=64, wait_ms=-1):
+        out = (Completion * max_n)()
+        n = C.c_int32()
+        check(lib().arma_hip_server_poll(self.h, out, max_n, C.byref(n), wait_ms))
+        return [out[i] for i in range(n.value)]
+
+    def release(self, rid):
+        check(lib().arma_hip_server_release(self.h, rid))
+
+    def copy_out(self, ptr, shape, dtype=np.float32):
+        """Host copy (numpy) of a device result the server owns."""
+        out = np.empty([int(v) for v in shape], dtype)
+        check(lib().arma_hip_server_copy(self.h, out.ctypes.data, C.c_void_p(ptr), out.nbytes))
+        return out
+
+    def stats(self):
+        st = ServerStats()
+        check(lib().arma_hip_server_stats(self.h, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.h:
+            lib().arma_hip_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <memory>
 #include <new>
 #include <string>
 #include <vector>
@@ -67,9 +68,68 @@ void recurrence(arma_hip_ctx* ctx, const arma_ar_desc* d, void* zeta) {
 	}
 }
 
+void check_config(const arma_batch_config& c) {
+	ARMA_HIP_REQUIRE(c.tile_x == 0 || c.tile_x == 8 || c.tile_x == 16 || c.tile_x == 32,
+	                 ARMA_HIP_EINVAL, "batch tile width must be 8, 16 or 32");
+	ARMA_HIP_REQUIRE(c.schedule >= ARMA_BATCH_AUTO && c.schedule <= ARMA_BATCH_INTERLEAVED,
+	                 ARMA_HIP_EINVAL, "bad batch schedule");
+	ARMA_HIP_REQUIRE(c.slots_per_cu >= 0, ARMA_HIP_EINVAL, "bad batch residency");
+}
+
+arma_hip::BatchArConfig to_internal(const arma_batch_config& c) {
+	arma_hip::BatchArConfig r;
+	r.tile_x = c.tile_x ? c.tile_x : 16;
+	r.schedule = c.schedule ? c.schedule : ARMA_BATCH_SURFACE_MAJOR;
+	r.slots_per_cu = c.slots_per_cu;
+	return r;
+}
+
 // mt.dat record: the raw mt_struct (15 four-byte fields, 4 bytes of padding
 // and the 8-byte state pointer on LP64) followed by nn state words.
 constexpr size_t kRecordHeader = 72;
+
+}  // namespace
+
+void arma_hip::set_last_error(const char* msg) { g_last_error = msg; }
+
+// A batch of surfaces: host copies of every descriptor, the arena layout,
+// and the device tables of its noise, recurrence and rescale launches.
+struct arma_hip_batch {
+	int precision = 0;
+	std::vector<arma_ar_desc> descs;
+	std::vector<std::vector<double>> phi;
+	std::vector<std::vector<arma_mt_params>> mts;
+	std::vector<std::vector<uint32_t>> seeds;
+	std::vector<arma_hip::BatchMember> members;
+	std::vector<int64_t> offsets;
+	std::string signature;
+	arma_hip::BatchArConfig cfg;
+	arma_hip::NoiseBatch* noise = nullptr;
+	arma_hip::ArBatch* ar = nullptr;
+	arma_hip::RescaleBatch* rescale = nullptr;
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+	void drop_graph() {
+		if (exec) (void)hipGraphExecDestroy(exec);
+		if (graph) (void)hipGraphDestroy(graph);
+		exec = nullptr;
+		graph = nullptr;
+	}
+	~arma_hip_batch() {
+		drop_graph();
+		if (noise) arma_hip::noise_batch_destroy(noise);
+		if (ar) arma_hip::ar_batch_destroy(ar);
+		if (rescale) arma_hip::rescale_batch_destroy(rescale);
+	}
+};
+
+namespace {
+
+void batch_enqueue(arma_hip_ctx* ctx, arma_hip_batch* b, void* arena) {
+	arma_hip::noise_batch_launch(ctx, b->noise, arena);
+	arma_hip::ar_batch_launch(ctx, b->ar, arena);
+	arma_hip::rescale_batch_launch(ctx, b->rescale, arena);
+}
 
 }  // namespace
 
@@ -320,6 +380,258 @@ int arma_hip_write_mt_config(const char* path, const arma_mt_params* params, int
 			}
 		}
 		ARMA_HIP_REQUIRE(bool(out), ARMA_HIP_ERUNTIME, "short write to mt.dat");
+	});
+}
+
+int arma_hip_batch_create(arma_hip_ctx* ctx, const arma_batch_desc* desc, arma_hip_batch** out) {
+	return guarded([&] {
+		use_device(ctx);
+		ARMA_HIP_REQUIRE(desc && out, ARMA_HIP_EINVAL, "null argument");
+		ARMA_HIP_REQUIRE(desc->n_surfaces > 0 && desc->surfaces, ARMA_HIP_EINVAL,
+		                 "empty batch");
+		ARMA_HIP_REQUIRE(desc->precision == ARMA_FP64 || desc->precision == ARMA_FP32,
+		                 ARMA_HIP_EINVAL, "bad precision");
+		check_config(desc->config);
+		std::unique_ptr<arma_hip_batch> b(new arma_hip_batch);
+		const int n = desc->n_surfaces;
+		b->precision = desc->precision;
+		b->descs.assign(desc->surfaces, desc->surfaces + n);
+		b->phi.resize(n);
+		b->mts.resize(n);
+		b->seeds.resize(n);
+		b->offsets.resize(n + 1);
+		int64_t off = 0;
+		char sig[64];
+		std::snprintf(sig, sizeof sig, "p%d n%d", desc->precision, n);
+		b->signature = sig;
+		for (int i = 0; i < n; ++i) {
+			arma_ar_desc& d = b->descs[i];
+			d.precision = desc->precision;
+			check_ar_desc(&d);
+			b->phi[i].assign(d.phi, d.phi + arma_hip::numel3(d.phi_shape));
+			d.phi = b->phi[i].data();
+			if (d.mts && d.n_mts > 0) {
+				b->mts[i].assign(d.mts, d.mts + d.n_mts);
+				d.mts = b->mts[i].data();
+				if (d.seeds) {
+					b->seeds[i].assign(d.seeds, d.seeds + d.n_mts);
+					d.seeds = b->seeds[i].data();
+				}
+			}
+			const double g0 = desc->gamma0 ? desc->gamma0[i] : 0.0;
+			const int h = desc->history ? desc->history[i] : 0;
+			ARMA_HIP_REQUIRE(h >= 0 && h < d.shape[0], ARMA_HIP_EINVAL, "bad history length");
+			b->members.push_back(arma_hip::BatchMember{size_t(off), &b->descs[i], g0, h});
+			b->offsets[i] = off;
+			// every surface starts on a 64-element boundary
+			off += int64_t((arma_hip::numel3(d.shape) + 63) / 64 * 64);
+			std::snprintf(sig, sizeof sig, " %d,%d,%d:%d,%d,%d", d.shape[0], d.shape[1],
+			              d.shape[2], d.phi_shape[0], d.phi_shape[1], d.phi_shape[2]);
+			b->signature += sig;
+		}
+		b->offsets[n] = off;
+		const arma_batch_config& c = desc->config;
+		if (!c.tile_x && !c.schedule && !c.slots_per_cu &&
+		    ctx->batch_tuned.count(b->signature)) {
+			const std::vector<int>& t = ctx->batch_tuned[b->signature];
+			b->cfg.tile_x = t[0];
+			b->cfg.schedule = t[1];
+			b->cfg.slots_per_cu = t[2];
+		} else {
+			b->cfg = to_internal(c);
+		}
+		b->noise = arma_hip::noise_batch_create(ctx, b->precision, b->members);
+		b->ar = arma_hip::ar_batch_create(ctx, b->precision, b->members, b->cfg);
+		b->rescale = arma_hip::rescale_batch_create(ctx, b->precision, b->members);
+		*out = b.release();
+	});
+}
+
+void arma_hip_batch_destroy(arma_hip_batch* batch) { delete batch; }
+
+int arma_hip_batch_layout(const arma_hip_batch* batch, int64_t* offsets) {
+	return guarded([&] {
+		ARMA_HIP_REQUIRE(batch && offsets, ARMA_HIP_EINVAL, "null argument");
+		std::memcpy(offsets, batch->offsets.data(), sizeof(int64_t) * batch->offsets.size());
+	});
+}
+
+int arma_hip_batch_generate(arma_hip_ctx* ctx, arma_hip_batch* batch, void* arena) {
+	return guarded([&] {
+		use_device(ctx);
+		ARMA_HIP_REQUIRE(batch && arena, ARMA_HIP_EINVAL, "null argument");
+		batch_enqueue(ctx, batch, arena);
+	});
+}
+
+int arma_hip_batch_capture(arma_hip_ctx* ctx, arma_hip_batch* batch, void* arena) {
+	return guarded([&] {
+		use_device(ctx);
+		ARMA_HIP_REQUIRE(batch && arena, ARMA_HIP_EINVAL, "null argument");
+		ARMA_HIP_REQUIRE(ctx->stream != nullptr, ARMA_HIP_EINVAL,
+		                 "graph capture needs a stream other than the null stream");
+		batch->drop_graph();
+		const bool prof = ctx->profiling;
+		ctx->profiling = false;  // no event records inside the graph
+		ARMA_HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+		hipGraph_t g = nullptr;
+		try {
+			batch_enqueue(ctx, batch, arena);
+		} catch (...) {
+			(void)hipStreamEndCapture(ctx->stream, &g);
+			if (g) (void)hipGraphDestroy(g);
+			ctx->profiling = prof;
+			throw;
+		}
+		ctx->profiling = prof;
+		ARMA_HIP_CHECK(hipStreamEndCapture(ctx->stream, &g));
+		batch->graph = g;
+		ARMA_HIP_CHECK(hipGraphInstantiate(&batch->exec, g, nullptr, nullptr, 0));
+	});
+}
+
+int arma_hip_batch_replay(arma_hip_ctx* ctx, arma_hip_batch* batch) {
+	return guarded([&] {
+		use_device(ctx);
+		ARMA_HIP_REQUIRE(batch && batch->exec, ARMA_HIP_EINVAL, "no captured graph");
+		arma_hip::KernelTimer timer(ctx, "batch_graph");
+		ARMA_HIP_CHECK(hipGraphLaunch(batch->exec, ctx->stream));
+	});
+}
+
+int arma_hip_batch_tune(arma_hip_ctx* ctx, arma_hip_batch* batch, void* arena, int32_t reps,
+                        arma_batch_trial* trials, int32_t max_trials, int32_t* n_trials) {
+	return guarded([&] {
+		use_device(ctx);
+		ARMA_HIP_REQUIRE(batch && arena, ARMA_HIP_EINVAL, "null argument");
+		reps = std::max(reps, 1);
+		hipEvent_t e0, e1;
+		ARMA_HIP_CHECK(hipEventCreate(&e0));
+		ARMA_HIP_CHECK(hipEventCreate(&e1));
+		struct Ev {
+			hipEvent_t a, b;
+			~Ev() {
+				(void)hipEventDestroy(a);
+				(void)hipEventDestroy(b);
+			}
+		} ev{e0, e1};
+		const bool prof = ctx->profiling;
+		ctx->profiling = false;
+		struct Restore {
+			arma_hip_ctx* c;
+			bool p;
+			~Restore() { c->profiling = p; }
+		} restore{ctx, prof};
+		std::vector<arma_hip::BatchArConfig> cands;
+		for (int tx : {16, 8, 32})
+			for (int sc : {ARMA_BATCH_SURFACE_MAJOR, ARMA_BATCH_INTERLEAVED})
+				for (int half = 0; half < 2; ++half) {
+					arma_hip::BatchArConfig c;
+					c.tile_x = tx;
+					c.schedule = sc;
+					c.slots_per_cu = half;  // resolved below: 0 = all that fit, 1 = half
+					cands.push_back(c);
+				}
+		batch->drop_graph();
+		arma_hip::ArBatch* best = nullptr;
+		arma_hip::BatchArConfig best_cfg;
+		double best_ms = 0.0;
+		int count = 0;
+		try {
+			for (arma_hip::BatchArConfig c : cands) {
+				if (!arma_hip::ar_batch_fits(batch->members, c.tile_x)) continue;
+				int want = 0;
+				if (c.slots_per_cu == 1) {
+					// half residency: only a candidate when more than one fits
+					arma_hip::BatchArConfig full = c;
+					full.slots_per_cu = 0;
+					std::unique_ptr<arma_hip::ArBatch, void (*)(arma_hip::ArBatch*)> probe(
+						arma_hip::ar_batch_create(ctx, batch->precision, batch->members, full),
+						arma_hip::ar_batch_destroy);
+					const int mx = arma_hip::ar_batch_max_slots(probe.get());
+					if (mx < 2) continue;
+					want = mx / 2;
+				}
+				c.slots_per_cu = want;
+				arma_hip::ArBatch* ab =
+					arma_hip::ar_batch_create(ctx, batch->precision, batch->members, c);
+				double ms = 1e300;
+				for (int r = 0; r < reps; ++r) {
+					arma_hip::noise_batch_launch(ctx, batch->noise, arena);
+					ARMA_HIP_CHECK(hipEventRecord(e0, ctx->stream));
+					arma_hip::ar_batch_launch(ctx, ab, arena);
+					ARMA_HIP_CHECK(hipEventRecord(e1, ctx->stream));
+					ARMA_HIP_CHECK(hipEventSynchronize(e1));
+					float t = 0.0f;
+					ARMA_HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
+					ms = std::min(ms, double(t));
+				}
+				if (trials && count < max_trials) {
+					arma_batch_trial& tr = trials[count];
+					tr.config.tile_x = c.tile_x;
+					tr.config.schedule = c.schedule;
+					tr.config.slots_per_cu = c.slots_per_cu;
+					tr.grid = arma_hip::ar_batch_grid(ab);
+					tr.ms = ms;
+				}
+				++count;
+				if (!best || ms < best_ms) {
+					if (best) arma_hip::ar_batch_destroy(best);
+					best = ab;
+					best_cfg = c;
+					best_ms = ms;
+				} else {
+					arma_hip::ar_batch_destroy(ab);
+				}
+			}
+		} catch (...) {
+			if (best) arma_hip::ar_batch_destroy(best);
+			throw;
+		}
+		if (n_trials) *n_trials = count;
+		ARMA_HIP_REQUIRE(best != nullptr, ARMA_HIP_ERUNTIME, "no batch configuration ran");
+		arma_hip::ar_batch_destroy(batch->ar);
+		batch->ar = best;
+		batch->cfg = best_cfg;
+		ctx->batch_tuned[batch->signature] = {best_cfg.tile_x, best_cfg.schedule,
+		                                      best_cfg.slots_per_cu};
+	});
+}
+
+int arma_hip_batch_get_config(const arma_hip_batch* batch, arma_batch_config* config,
+                              int32_t* grid) {
+	return guarded([&] {
+		ARMA_HIP_REQUIRE(batch, ARMA_HIP_EINVAL, "null argument");
+		if (config) {
+			config->tile_x = batch->cfg.tile_x;
+			config->schedule = batch->cfg.schedule;
+			config->slots_per_cu = batch->cfg.slots_per_cu;
+		}
+		if (grid) *grid = arma_hip::ar_batch_grid(batch->ar);
+	});
+}
+
+int arma_hip_batch_set_config(arma_hip_ctx* ctx, arma_hip_batch* batch,
+                              const arma_batch_config* config) {
+	return guarded([&] {
+		use_device(ctx);
+		ARMA_HIP_REQUIRE(batch && config, ARMA_HIP_EINVAL, "null argument");
+		check_config(*config);
+		const arma_hip::BatchArConfig c = to_internal(*config);
+		arma_hip::ArBatch* ab = arma_hip::ar_batch_create(ctx, batch->precision, batch->members, c);
+		batch->drop_graph();
+		ARMA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+		arma_hip::ar_batch_destroy(batch->ar);
+		batch->ar = ab;
+		batch->cfg = c;
+	});
+}
+
+int arma_hip_batch_factors(arma_hip_ctx* ctx, arma_hip_batch* batch, double* out) {
+	return guarded([&] {
+		use_device(ctx);
+		ARMA_HIP_REQUIRE(batch && out, ARMA_HIP_EINVAL, "null argument");
+		arma_hip::rescale_batch_factors(ctx, batch->rescale, out);
 	});
 }
 

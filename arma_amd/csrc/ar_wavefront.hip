@@ -45,9 +45,10 @@
 namespace arma_hip {
 namespace {
 
-constexpr int kTX = 16;
-constexpr int kTY = 16;
-constexpr int kThreads = kTX * kTY;
+// Tiles are TX x TY columns with TX * TY = kThreads: 8 x 32, 16 x 16 or
+// 32 x 8 (the halo of a 10 x 10 in-slice order then needs at most two slots
+// per lane).
+constexpr int kThreads = 256;
 constexpr int kD = 4;  // prefetch distance of the noise and halo loads, in steps
 constexpr int kMaxP12 = 10;  // largest P1, P2 handled
 constexpr int kProgStride = 32;  // ints between tile progress counters: one per 128-byte line
@@ -124,24 +125,31 @@ __host__ __device__ constexpr int tap_chunks(int p12) { return (p12 - 1 + kCH - 
 // Mailbox column of tile-local lane (x, y), or -1 when no neighbour reads it:
 // the bottom HX rows first (row-major), then the right HY columns of the
 // rows above them.
+template <int TX>
 __host__ __device__ __forceinline__ int mbox_column(int x, int y, int HX, int HY) {
-	if (x >= kTX - HX) return (x - (kTX - HX)) * kTY + y;
-	if (y >= kTY - HY) return kTY * HX + x * HY + (y - (kTY - HY));
+	constexpr int TY = kThreads / TX;
+	if (x >= TX - HX) return (x - (TX - HX)) * TY + y;
+	if (y >= TY - HY) return TY * HX + x * HY + (y - (TY - HY));
 	return -1;
 }
 
+template <int TX>
 __host__ __device__ constexpr int mbox_width(int HX, int HY) {
-	return kTY * HX + (kTX - HX) * HY;
+	return (kThreads / TX) * HX + (TX - HX) * HY;
 }
+
+inline int mbox_width_rt(int TX, int HX, int HY) { return (kThreads / TX) * HX + (TX - HX) * HY; }
 
 // The march of one tile over all of t (P0 accumulators; a surface of lower
 // order in t has zero coefficients past its own order).
-template <class T, int P0, bool KAHAN>
+template <class T, int TX, int P0, bool KAHAN>
 __device__ __forceinline__ void wave_tile(
 	T* __restrict__ zeta, int nt, int nx, int ny, const T* __restrict__ phi0,
 	const T* __restrict__ phic, const int* __restrict__ toff, int P1, int P2, int nY,
 	int* __restrict__ progress, const T* __restrict__ zero, T* __restrict__ sink,
-	T* __restrict__ mbox, int NB, int total_steps, int b, unsigned char* smem_raw) {
+	T* __restrict__ mbox, int NB, int total_steps, int t0, int b, unsigned char* smem_raw) {
+	constexpr int kTX = TX, kTY = kThreads / TX;
+	static_assert(kTX * kTY == kThreads && kTX >= 8 && kTY >= 8, "tile shape");
 	const int P12 = P1 * P2;
 	T* ring = reinterpret_cast<T*>(smem_raw);
 	const int HX = P1 - 1, HY = P2 - 1;
@@ -191,7 +199,7 @@ __device__ __forceinline__ void wave_tile(
 			// at u-step u + 16 (edge neighbours) or u + 32 (corner)
 			const int px = hx < 0 ? kTX + hx : hx, py = hy < 0 ? kTY + hy : hy;
 			hnb[r] = b - (hx < 0 ? nY : 0) - (hy < 0 ? 1 : 0);
-			hm[r] = mbox_column(px, py, HX, HY);
+			hm[r] = mbox_column<TX>(px, py, HX, HY);
 			hdu[r] = (hx < 0 ? kTX : 0) + (hy < 0 ? kTY : 0);
 		}
 	}
@@ -236,7 +244,7 @@ __device__ __forceinline__ void wave_tile(
 	// value into this tile's mailbox, row u, column my_m: whole rows per
 	// step, so the device-coherent stores are contiguous.  The surface
 	// itself is stored with ordinary (write-back) stores.
-	const int my_m = mbox_column(lx, ly, HX, HY);
+	const int my_m = mbox_column<TX>(lx, ly, HX, HY);
 	T* const my_mbox = my_m >= 0 ? mbox + size_t(b) * total_u * NB + my_m : my_sink;
 	const size_t mstep = my_m >= 0 ? size_t(NB) : 0;
 	auto issue_loads = [&](int u2, T& e, T& h0, T& h1) {
@@ -307,8 +315,11 @@ __device__ __forceinline__ void wave_tile(
 				}
 			}
 			acc[0].add(part[0]);
+			// slices t < t0 are given (a realtime stream's history): they
+			// feed the sums of later slices unchanged
 			const T eps = e_q;
-			const T z = eps + (acc[0].value() + phi000 * eps);
+			const T zc = eps + (acc[0].value() + phi000 * eps);
+			const T z = t < t0 ? eps : zc;
 			if (live) ring[rs_own * slot + own_lds] = z;
 			*(live ? zeta + size_t(t) * sxy + own_off : my_sink) = z;
 			coherent_store(my_mbox + size_t(u) * mstep, z);
@@ -364,15 +375,123 @@ __device__ __forceinline__ void wave_tile(
 // Kernel arguments stay plain restrict-qualified pointers: the coefficient
 // reads are then provably unclobbered by the kernel's stores and compile to
 // scalar loads (a struct argument costs 30-60 VGPRs per lane).
-template <class T, int P0, bool KAHAN>
+template <class T, int TX, int P0, bool KAHAN>
 __global__ __launch_bounds__(kThreads) void ar_wave_kernel(
 	T* __restrict__ zeta, int nt, int nx, int ny, const T* __restrict__ phi0,
 	const T* __restrict__ phic, const int* __restrict__ toff, int P1, int P2, int nY,
 	int* __restrict__ progress, const T* __restrict__ zero, T* __restrict__ sink,
 	T* __restrict__ mbox, int NB, int total_steps) {
 	extern __shared__ __align__(16) unsigned char smem_raw[];
-	wave_tile<T, P0, KAHAN>(zeta, nt, nx, ny, phi0, phic, toff, P1, P2, nY, progress, zero, sink,
-	                        mbox, NB, total_steps, blockIdx.x, smem_raw);
+	wave_tile<T, TX, P0, KAHAN>(zeta, nt, nx, ny, phi0, phic, toff, P1, P2, nY, progress, zero, sink,
+	                            mbox, NB, total_steps, 0, blockIdx.x, smem_raw);
+}
+
+// ---- many surfaces in one persistent launch ----
+//
+// Each surface keeps the layout of a single-surface launch (its own tiles,
+// progress counters and mailboxes, at offsets in shared arenas).  A queue
+// lists (surface, tile) pairs in an order where every tile comes after the
+// tiles it waits on; each workgroup marches its share of the entries in
+// queue order.  The coefficients of a surface are padded with
+// zeros in k to the nearest compiled P0; the padded terms add exact zeros, so
+// a surface's values do not depend on the padding, the tile shape or the
+// schedule.
+
+struct BatchSurface {
+	long long zoff;  // first element of the surface in the arena
+	long long moff;  // first element of its mailboxes
+	int nt, nx, ny;
+	int P1, P2;
+	int bucket;  // index into kBatchP0
+	int nY;      // tiles along y
+	int coff;    // coefficient block (elements): phi0 block, then the chunks
+	int n0;      // length of the phi0 block
+	int toff;    // tap offset table (ints)
+	int poff;    // first progress counter (tiles)
+	int NB, total_steps;
+	int t0;  // leading slices that are given, not generated
+	int pad;
+};
+
+constexpr int kBatchP0[] = {2, 3, 5, 7, 9, 10};
+constexpr int kBatchBuckets = 6;
+
+int batch_bucket(int P0) {
+	for (int i = 0; i < kBatchBuckets; ++i)
+		if (P0 <= kBatchP0[i]) return i;
+	return -1;
+}
+
+// Workgroup w takes queue entries w, w + G, w + 2G, ... in order (G
+// workgroups, all resident).  The earliest unfinished entry then always
+// belongs to a workgroup that has finished its earlier entries and is
+// marching that one, and it only waits on earlier entries, which are done:
+// the launch cannot deadlock.  The loop control is uniform (no atomic
+// fetch under a thread-0 branch), which keeps the compiled loop a plain
+// scalar loop around the tile's barriers.
+template <class T, int TX, bool KAHAN>
+__global__ __launch_bounds__(kThreads) void ar_batch_kernel(
+	T* __restrict__ arena, const T* __restrict__ coef, const int* __restrict__ taps,
+	int* __restrict__ progress, const T* __restrict__ zero, T* __restrict__ sink,
+	T* __restrict__ mbox, const BatchSurface* __restrict__ surf, const int2* __restrict__ queue,
+	int nqueue) {
+	extern __shared__ __align__(16) unsigned char smem_raw[];
+	for (int item = blockIdx.x; item < nqueue; item += gridDim.x) {
+		const int si = queue[item].x;
+		const int b = queue[item].y;
+		const BatchSurface& sf = surf[si];
+		T* const z = arena + sf.zoff;
+		const T* const c0 = coef + sf.coff;
+		const T* const cc = c0 + sf.n0;
+		const int* const tp = taps + sf.toff;
+		int* const pr = progress + size_t(sf.poff) * kProgStride;
+		T* const mb = mbox + sf.moff;
+		switch (sf.bucket) {
+#define ARMA_BATCH_BODY(I)                                                                    \
+	case I:                                                                                   \
+		wave_tile<T, TX, kBatchP0[I], KAHAN>(z, sf.nt, sf.nx, sf.ny, c0, cc, tp, sf.P1, sf.P2,  \
+		                                     sf.nY, pr, zero, sink, mb, sf.NB, sf.total_steps, \
+		                                     sf.t0, b, smem_raw);                              \
+		break;
+			ARMA_BATCH_BODY(0)
+			ARMA_BATCH_BODY(1)
+			ARMA_BATCH_BODY(2)
+			ARMA_BATCH_BODY(3)
+			ARMA_BATCH_BODY(4)
+			ARMA_BATCH_BODY(5)
+#undef ARMA_BATCH_BODY
+		}
+	}
+}
+
+// Coefficient and tap tables of one surface, appended to ph and offs.
+// ph gets the phi0 block (n0 elements: phi(k,0,0), zero past P0) and then
+// the in-slice taps as [chunk][q][k] with k stride P0S (the P0 the kernel is
+// compiled for; zero for k >= P0): tap m = i*P2 + j = 1 + chunk*kCH + q.
+// offs gets each tap's LDS byte distance back from the own position for a
+// ring row of WY slots (zero, i.e. the own position, past the last tap).
+template <class T>
+int append_tables(const double* phi, int P0, int P1, int P2, int P0S, int WY,
+                  std::vector<T>& ph, std::vector<int>& offs) {
+	const int P12 = P1 * P2;
+	const int nch = tap_chunks(P12);
+	const int n0 = (P0S + 15) / 16 * 16;
+	const size_t base = ph.size();
+	ph.resize(base + n0 + size_t(nch) * P0S * kCH, T(0));
+	for (int k = 0; k < P0; ++k) {
+		const double* pk = phi + size_t(k) * P12;
+		ph[base + k] = T(pk[0]);
+		for (int m = 1; m < P12; ++m) {
+			const int ch = (m - 1) / kCH, q = (m - 1) % kCH;
+			ph[base + n0 + (size_t(ch) * kCH + q) * P0S + k] = T(pk[m]);
+		}
+	}
+	// keep every surface's block 16-element aligned
+	ph.resize((ph.size() + 15) / 16 * 16, T(0));
+	const size_t ob = offs.size();
+	offs.resize(ob + size_t(nch) * kCH, 0);
+	for (int m = 1; m < P12; ++m) offs[ob + m - 1] = ((m / P2) * WY + (m % P2)) * int(sizeof(T));
+	return n0;
 }
 
 template <class T, bool KAHAN>
@@ -381,24 +500,10 @@ void launch_typed(arma_hip_ctx* ctx, const arma_ar_desc* d, T* zeta) {
 	const int P0 = d->phi_shape[0], P1 = d->phi_shape[1], P2 = d->phi_shape[2];
 	ARMA_HIP_REQUIRE(P1 <= kMaxP12 && P2 <= kMaxP12 && P0 <= 10, ARMA_HIP_EINVAL,
 	                 "AR order above 10 is not supported by the wavefront kernel");
-	// phi0[k] = phi(k,0,0); phic[ch][k][q] = phi(k, m) for the in-slice tap
-	// m = i*P2 + j = 1 + ch*kCH + q (zero past the last tap)
-	const int P12 = P1 * P2;
-	const int nch = tap_chunks(P12);
-	const size_t n0 = (size_t(P0) + 15) / 16 * 16;
-	std::vector<T> ph(n0 + size_t(nch) * P0 * kCH, T(0));
-	for (int k = 0; k < P0; ++k) {
-		const double* pk = d->phi + size_t(k) * P12;
-		ph[k] = T(pk[0]);
-		for (int m = 1; m < P12; ++m) {
-			const int ch = (m - 1) / kCH, q = (m - 1) % kCH;
-			ph[n0 + (size_t(ch) * kCH + q) * P0 + k] = T(pk[m]);
-		}
-	}
-	// tap offsets in LDS bytes, back from the own position: (i*WY + j) slots
-	const int WY = kTY + P2 - 1;
-	std::vector<int> offs(size_t(nch) * kCH, 0);
-	for (int m = 1; m < P12; ++m) offs[m - 1] = ((m / P2) * WY + (m % P2)) * int(sizeof(T));
+	constexpr int kTX = 16, kTY = kThreads / kTX;
+	std::vector<T> ph;
+	std::vector<int> offs;
+	const size_t n0 = append_tables(d->phi, P0, P1, P2, P0, kTY + P2 - 1, ph, offs);
 	const size_t phi_bytes = (ph.size() * sizeof(T) + 255) / 256 * 256;
 	unsigned char* dbuf = (unsigned char*)ctx->phi.get(phi_bytes + offs.size() * sizeof(int) + 4);
 	T* dphi = (T*)dbuf;
@@ -418,7 +523,7 @@ void launch_typed(arma_hip_ctx* ctx, const arma_ar_desc* d, T* zeta) {
 	const T* zero = (const T*)(fb + prog_bytes);
 	T* sink = (T*)(fb + prog_bytes + zero_bytes);
 	// mailboxes: one row of boundary values per tile and step
-	const int NB = std::max(1, mbox_width(P1 - 1, P2 - 1));
+	const int NB = std::max(1, mbox_width<kTX>(P1 - 1, P2 - 1));
 	const int total_u = total_steps + (P1 - 1) + (P2 - 1);
 	T* mbox = (T*)ctx->mbox.get(sizeof(T) * size_t(ntiles) * total_u * NB);
 	const int nclear = int((prog_bytes + zero_bytes) / sizeof(int));
@@ -433,9 +538,9 @@ void launch_typed(arma_hip_ctx* ctx, const arma_ar_desc* d, T* zeta) {
 	case P:                                                                                   \
 		if (lds > 65536)                                                                      \
 			ARMA_HIP_CHECK(hipFuncSetAttribute(                                               \
-				reinterpret_cast<const void*>(&ar_wave_kernel<T, P, KAHAN>),                  \
+				reinterpret_cast<const void*>(&ar_wave_kernel<T, kTX, P, KAHAN>),             \
 				hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));                       \
-		ar_wave_kernel<T, P, KAHAN><<<ntiles, kThreads, lds, ctx->stream>>>(                  \
+		ar_wave_kernel<T, kTX, P, KAHAN><<<ntiles, kThreads, lds, ctx->stream>>>(             \
 			zeta, nt, nx, ny, dphi, dphi + n0, doffs, P1, P2, nY, prog, zero, sink, mbox, NB, \
 			total_steps);                                                                     \
 		break;
@@ -457,6 +562,156 @@ void launch_typed(arma_hip_ctx* ctx, const arma_ar_desc* d, T* zeta) {
 	ARMA_HIP_CHECK(hipGetLastError());
 }
 
+template <class T, int TX, bool KAHAN>
+struct BatchKernel {
+	static const void* fn() { return reinterpret_cast<const void*>(&ar_batch_kernel<T, TX, KAHAN>); }
+};
+
+const void* batch_kernel_fn(int precision, int tx) {
+	if (precision == ARMA_FP32) {
+		if (tx == 8) return BatchKernel<float, 8, true>::fn();
+		if (tx == 32) return BatchKernel<float, 32, true>::fn();
+		return BatchKernel<float, 16, true>::fn();
+	}
+	if (tx == 8) return BatchKernel<double, 8, false>::fn();
+	if (tx == 32) return BatchKernel<double, 32, false>::fn();
+	return BatchKernel<double, 16, false>::fn();
+}
+
+}  // namespace
+
+struct ArBatch {
+	int precision = 0;
+	BatchArConfig cfg;
+	int nqueue = 0, grid = 0, max_slots = 0;
+	size_t lds = 0;
+	size_t nclear = 0;  // ints cleared before a launch: queue counter, progress, zero word
+	Scratch coef, taps, surf, queue, flags, mbox;
+	int* clear_base = nullptr;  // start of the words cleared before a launch
+	int* progress = nullptr;
+	void* zero = nullptr;
+	void* sink = nullptr;
+};
+
+namespace {
+
+template <class T>
+void build_batch(arma_hip_ctx* ctx, ArBatch* ab, const std::vector<BatchMember>& m) {
+	const int TX = ab->cfg.tile_x, TY = kThreads / TX;
+	std::vector<T> coef;
+	std::vector<int> taps;
+	std::vector<BatchSurface> surf(m.size());
+	std::vector<double> work(m.size());
+	std::vector<int> ntiles(m.size());
+	long long tiles = 0, mbox_elems = 0;
+	int max_tiles = 0;
+	size_t lds = 0;
+	for (size_t i = 0; i < m.size(); ++i) {
+		const arma_ar_desc* d = m[i].d;
+		const int nt = d->shape[0], nx = d->shape[1], ny = d->shape[2];
+		const int P0 = d->phi_shape[0], P1 = d->phi_shape[1], P2 = d->phi_shape[2];
+		ARMA_HIP_REQUIRE(P1 <= kMaxP12 && P2 <= kMaxP12 && P0 <= 10, ARMA_HIP_EINVAL,
+		                 "AR order above 10 is not supported by the wavefront kernel");
+		// a tile's halo comes from its direct neighbours only
+		ARMA_HIP_REQUIRE(P1 - 1 <= TX && P2 - 1 <= TY, ARMA_HIP_EINVAL,
+		                 "AR order wider than the batch tile");
+		const int bucket = batch_bucket(P0);
+		BatchSurface& sf = surf[i];
+		sf.zoff = (long long)m[i].zoff;
+		sf.nt = nt;
+		sf.nx = nx;
+		sf.ny = ny;
+		sf.P1 = P1;
+		sf.P2 = P2;
+		sf.bucket = bucket;
+		sf.nY = div_ceil(ny, TY);
+		sf.coff = int(coef.size());
+		sf.toff = int(taps.size());
+		sf.n0 = append_tables(d->phi, P0, P1, P2, kBatchP0[bucket], TY + P2 - 1, coef, taps);
+		ntiles[i] = div_ceil(nx, TX) * sf.nY;
+		sf.poff = int(tiles);
+		sf.total_steps = nt + TX + TY - 2;
+		sf.t0 = m[i].t0;
+		sf.pad = 0;
+		sf.NB = std::max(1, mbox_width_rt(TX, P1 - 1, P2 - 1));
+		const long long total_u = sf.total_steps + (P1 - 1) + (P2 - 1);
+		sf.moff = mbox_elems;
+		mbox_elems += (long long)ntiles[i] * total_u * sf.NB;
+		mbox_elems = (mbox_elems + 63) / 64 * 64;
+		tiles += ntiles[i];
+		max_tiles = std::max(max_tiles, ntiles[i]);
+		lds = std::max(lds, size_t(P1 + P2 - 1) * (TX + P1 - 1) * (TY + P2 - 1) * sizeof(T));
+		work[i] = double(nt) * nx * ny * kBatchP0[bucket] * P1 * P2;
+	}
+	ARMA_HIP_REQUIRE(lds <= 160 * 1024, ARMA_HIP_EINVAL, "AR order too large for the LDS ring");
+	ARMA_HIP_REQUIRE(tiles < (1ll << 31) / kProgStride && coef.size() < (1u << 31) &&
+	                     taps.size() < (1u << 31),
+	                 ARMA_HIP_EINVAL, "batch too large");
+	// the queue: surfaces by decreasing work; tile rows in order inside a
+	// surface, so every tile follows the tiles it waits on
+	std::vector<int> order(m.size());
+	for (size_t i = 0; i < m.size(); ++i) order[i] = int(i);
+	std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
+	std::vector<int2> queue;
+	queue.reserve(size_t(tiles));
+	if (ab->cfg.schedule == ARMA_BATCH_INTERLEAVED) {
+		std::vector<int> done(m.size(), 0);
+		for (bool more = true; more;) {
+			more = false;
+			for (int i : order) {
+				const int row = surf[i].nY;
+				for (int k = 0; k < row && done[i] < ntiles[i]; ++k, ++done[i])
+					queue.push_back(make_int2(i, done[i]));
+				more = more || done[i] < ntiles[i];
+			}
+		}
+	} else {
+		for (int i : order)
+			for (int b = 0; b < ntiles[i]; ++b) queue.push_back(make_int2(i, b));
+	}
+	ab->nqueue = int(queue.size());
+	ab->lds = lds;
+	// device tables
+	ab->coef.get(std::max<size_t>(coef.size(), 16) * sizeof(T));
+	ab->taps.get(std::max<size_t>(taps.size(), 16) * sizeof(int));
+	ab->surf.get(surf.size() * sizeof(BatchSurface));
+	ab->queue.get(std::max<size_t>(queue.size(), 1) * sizeof(int2));
+	upload(ctx, ab->coef.ptr, coef.data(), coef.size() * sizeof(T));
+	upload(ctx, ab->taps.ptr, taps.data(), taps.size() * sizeof(int));
+	upload(ctx, ab->surf.ptr, surf.data(), surf.size() * sizeof(BatchSurface));
+	upload(ctx, ab->queue.ptr, queue.data(), queue.size() * sizeof(int2));
+	// flags: the queue counter (own line), progress counters, a zero word,
+	// and a sink for out-of-range stores sized for the largest surface
+	const size_t head = 256, prog_bytes = sizeof(int) * kProgStride * size_t(tiles);
+	const size_t zero_bytes = 256;
+	unsigned char* fb = (unsigned char*)ab->flags.get(head + prog_bytes + zero_bytes +
+	                                                  size_t(max_tiles) * kThreads * sizeof(T));
+	ab->clear_base = (int*)fb;
+	ab->progress = (int*)(fb + head);
+	ab->zero = fb + head + prog_bytes;
+	ab->sink = fb + head + prog_bytes + zero_bytes;
+	ab->nclear = (head + prog_bytes + zero_bytes) / sizeof(int);
+	ab->mbox.get(std::max<size_t>(size_t(mbox_elems), 64) * sizeof(T));
+	const void* fn = batch_kernel_fn(ab->precision, TX);
+	if (lds > 65536)
+		ARMA_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+		                                   int(lds)));
+	int per_cu = 0;
+	ARMA_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kThreads, lds));
+	ab->max_slots = std::max(per_cu, 1);
+	const int slots = ab->cfg.slots_per_cu > 0 ? std::min(ab->cfg.slots_per_cu, ab->max_slots)
+	                                           : ab->max_slots;
+	ab->grid = int(std::min<long long>(ab->nqueue, (long long)slots * std::max(ctx->num_cus, 1)));
+}
+
+template <class T, int TX, bool KAHAN>
+void batch_go(arma_hip_ctx* ctx, ArBatch* ab, T* arena) {
+	ar_batch_kernel<T, TX, KAHAN><<<ab->grid, kThreads, ab->lds, ctx->stream>>>(
+		arena, (const T*)ab->coef.ptr, (const int*)ab->taps.ptr, ab->progress,
+		(const T*)ab->zero, (T*)ab->sink, (T*)ab->mbox.ptr, (const BatchSurface*)ab->surf.ptr,
+		(const int2*)ab->queue.ptr, ab->nqueue);
+}
+
 }  // namespace
 
 void launch_ar_wavefront(arma_hip_ctx* ctx, const arma_ar_desc* d, void* zeta) {
@@ -465,5 +720,76 @@ void launch_ar_wavefront(arma_hip_ctx* ctx, const arma_ar_desc* d, void* zeta) {
 	else
 		launch_typed<double, false>(ctx, d, (double*)zeta);
 }
+
+bool ar_batch_fits(const std::vector<BatchMember>& m, int tile_x) {
+	for (const BatchMember& x : m)
+		if (x.d->phi_shape[1] - 1 > tile_x || x.d->phi_shape[2] - 1 > kThreads / tile_x)
+			return false;
+	return true;
+}
+
+void ar_batch_build(arma_hip_ctx* ctx, ArBatch* ab, int precision,
+                    const std::vector<BatchMember>& m, const BatchArConfig& cfg) {
+	ARMA_HIP_REQUIRE(cfg.tile_x == 8 || cfg.tile_x == 16 || cfg.tile_x == 32, ARMA_HIP_EINVAL,
+	                 "batch tile width must be 8, 16 or 32");
+	ARMA_HIP_REQUIRE(cfg.schedule == ARMA_BATCH_SURFACE_MAJOR ||
+	                     cfg.schedule == ARMA_BATCH_INTERLEAVED,
+	                 ARMA_HIP_EINVAL, "bad batch schedule");
+	ARMA_HIP_REQUIRE(!m.empty(), ARMA_HIP_EINVAL, "empty batch");
+	ab->precision = precision;
+	ab->cfg = cfg;
+	if (precision == ARMA_FP32)
+		build_batch<float>(ctx, ab, m);
+	else
+		build_batch<double>(ctx, ab, m);
+}
+
+ArBatch* ar_batch_create(arma_hip_ctx* ctx, int precision, const std::vector<BatchMember>& m,
+                         const BatchArConfig& cfg) {
+	auto* ab = new ArBatch;
+	try {
+		ar_batch_build(ctx, ab, precision, m, cfg);
+	} catch (...) {
+		delete ab;
+		throw;
+	}
+	return ab;
+}
+
+ArBatch* ar_batch_new() { return new ArBatch; }
+
+size_t ar_batch_member_bytes(const arma_ar_desc* d, int tile_x) {
+	const int TX = tile_x, TY = kThreads / TX;
+	const size_t es = d->precision == ARMA_FP32 ? 4 : 8;
+	const int P1 = d->phi_shape[1], P2 = d->phi_shape[2];
+	const size_t tiles = size_t(div_ceil(d->shape[1], TX)) * div_ceil(d->shape[2], TY);
+	const size_t total_u = size_t(d->shape[0]) + TX + TY - 2 + (P1 - 1) + (P2 - 1);
+	const size_t NB = std::max(1, mbox_width_rt(TX, P1 - 1, P2 - 1));
+	return es * (tiles * total_u * NB + tiles * kThreads) + sizeof(int) * kProgStride * tiles;
+}
+
+void ar_batch_launch(arma_hip_ctx* ctx, ArBatch* ab, void* arena) {
+	const int nclear = int(ab->nclear);
+	reset_progress<<<div_ceil(nclear, 256), 256, 0, ctx->stream>>>(ab->clear_base, nclear);
+	ARMA_HIP_CHECK(hipGetLastError());
+	KernelTimer timer(ctx, "ar_batch");
+	const int tx = ab->cfg.tile_x;
+	if (ab->precision == ARMA_FP32) {
+		if (tx == 8) batch_go<float, 8, true>(ctx, ab, (float*)arena);
+		else if (tx == 32) batch_go<float, 32, true>(ctx, ab, (float*)arena);
+		else batch_go<float, 16, true>(ctx, ab, (float*)arena);
+	} else {
+		if (tx == 8) batch_go<double, 8, false>(ctx, ab, (double*)arena);
+		else if (tx == 32) batch_go<double, 32, false>(ctx, ab, (double*)arena);
+		else batch_go<double, 16, false>(ctx, ab, (double*)arena);
+	}
+	ARMA_HIP_CHECK(hipGetLastError());
+}
+
+void ar_batch_destroy(ArBatch* ab) { delete ab; }
+
+int ar_batch_grid(const ArBatch* ab) { return ab->grid; }
+
+int ar_batch_max_slots(const ArBatch* ab) { return ab->max_slots; }
 
 }  // namespace arma_hip

@@ -54,6 +54,10 @@ struct Scratch {
 		ptr = nullptr;
 		bytes = 0;
 	}
+	Scratch() = default;
+	Scratch(const Scratch&) = delete;
+	Scratch& operator=(const Scratch&) = delete;
+	~Scratch() { release(); }
 };
 
 }  // namespace arma_hip
@@ -86,6 +90,9 @@ struct arma_hip_ctx {
 	arma_hip::Scratch params, seeds, state, phi, partials, flags, noise_tmp, mbox;
 	arma_hip::Scratch pot_spec, pot_mid, pot_wn, pot_twiddle, pot_kmin, zeta_tmp, phi_tmp;
 	int twiddle_n = 0;
+	// batch configurations chosen by the autotuner, by batch signature:
+	// tile width, schedule, workgroups per CU
+	std::map<std::string, std::vector<int>> batch_tuned;
 };
 
 namespace arma_hip {
@@ -121,6 +128,9 @@ inline void upload(arma_hip_ctx* ctx, void* dst, const void* src, size_t bytes) 
 	ARMA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
 }
 
+// Records msg as the calling thread's last error (arma_hip_last_error).
+void set_last_error(const char* msg);
+
 // Kernel launchers implemented in the .hip translation units.
 void launch_white_noise(arma_hip_ctx* ctx, const arma_ar_desc* d, void* eps);
 void launch_mt_raw(arma_hip_ctx* ctx, const arma_mt_params* mts, const uint32_t* seeds,
@@ -129,6 +139,72 @@ void launch_ar_plain(arma_hip_ctx* ctx, const arma_ar_desc* d, void* zeta);
 void launch_ar_wavefront(arma_hip_ctx* ctx, const arma_ar_desc* d, void* zeta);
 double launch_rescale(arma_hip_ctx* ctx, void* zeta, const int32_t shape[3], int precision,
                       double gamma0);
+
+// ---- batches of surfaces in one arena ----
+
+// The output block of one noise stream inside a batch arena.
+struct NoiseBlockRec {
+	long long base;  // arena element of the surface's (0, 0, 0)
+	int sy, sz;      // surface extents along x and y
+	int lo0, lo1, lo2;
+	int e0, e1, e2;
+	float sdf;
+	double sd;
+};
+
+// One surface of a batch: its first arena element and its generator.
+struct BatchMember {
+	size_t zoff;
+	const arma_ar_desc* d;
+	double gamma0;  // post-rescale target variance; <= 0 skips the rescale
+	int t0;         // leading slices given by the caller (noise and values start at t0)
+};
+
+struct BatchArConfig {
+	int tile_x = 16;       // 8, 16 or 32; tiles are tile_x x (256 / tile_x)
+	int schedule = ARMA_BATCH_SURFACE_MAJOR;
+	int slots_per_cu = 0;  // resident workgroups per CU, 0 = as many as fit
+};
+
+struct NoiseBatch;
+struct ArBatch;
+struct RescaleBatch;
+// Construction uploads every table and allocates every buffer; the launches
+// only enqueue kernels on the context stream (no allocation, no host wait),
+// so they can be captured into a graph.
+NoiseBatch* noise_batch_create(arma_hip_ctx* ctx, int precision,
+                               const std::vector<BatchMember>& m);
+void noise_batch_launch(arma_hip_ctx* ctx, NoiseBatch* b, void* arena);
+void noise_batch_destroy(NoiseBatch* b);
+ArBatch* ar_batch_create(arma_hip_ctx* ctx, int precision, const std::vector<BatchMember>& m,
+                         const BatchArConfig& cfg);
+void ar_batch_launch(arma_hip_ctx* ctx, ArBatch* b, void* arena);
+void ar_batch_destroy(ArBatch* b);
+int ar_batch_grid(const ArBatch* b);
+bool ar_batch_fits(const std::vector<BatchMember>& m, int tile_x);
+int ar_batch_max_slots(const ArBatch* b);
+RescaleBatch* rescale_batch_create(arma_hip_ctx* ctx, int precision,
+                                   const std::vector<BatchMember>& m);
+void rescale_batch_launch(arma_hip_ctx* ctx, RescaleBatch* b, void* arena);
+// Factors of the last launch, one per member (0 for members without a
+// rescale); waits for the stream.
+void rescale_batch_factors(arma_hip_ctx* ctx, RescaleBatch* b, double* out);
+void rescale_batch_destroy(RescaleBatch* b);
+// Empty objects and rebuilds in place: a rebuild reuses the device buffers
+// of earlier builds (they only grow), so a serving lane allocates nothing
+// once warm.
+NoiseBatch* noise_batch_new();
+void noise_batch_build(arma_hip_ctx* ctx, NoiseBatch* b, int precision,
+                       const std::vector<BatchMember>& m);
+ArBatch* ar_batch_new();
+void ar_batch_build(arma_hip_ctx* ctx, ArBatch* b, int precision,
+                    const std::vector<BatchMember>& m, const BatchArConfig& cfg);
+RescaleBatch* rescale_batch_new();
+void rescale_batch_build(arma_hip_ctx* ctx, RescaleBatch* b, int precision,
+                         const std::vector<BatchMember>& m);
+// Device bytes a member adds to a recurrence launch (mailboxes, sink,
+// progress counters).
+size_t ar_batch_member_bytes(const arma_ar_desc* d, int tile_x);
 void launch_linear_potential(arma_hip_ctx* ctx, const arma_pot_desc* d, const void* zeta,
                              void* phi, double* wnmax_out);
 

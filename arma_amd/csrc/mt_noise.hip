@@ -151,38 +151,81 @@ struct PolarF {
 	}
 };
 
+// The block of the output one stream fills: its origin and extents inside a
+// surface, the surface's row lengths and the standard deviation.
+template <class OutT>
+struct StreamBlock {
+	OutT* out;   // element (0, 0, 0) of the surface
+	int sy, sz;  // surface extents along x and y
+	int lo0, lo1, lo2;
+	int e0, e1, e2;
+	double sd;
+	float sdf;
+};
+
+// One surface: stream p fills partition p (row-major partition index).
+template <class OutT>
+struct GridBlocks {
+	OutT* out;
+	int3 shape, part, np;
+	double sd;
+	float sdf;
+	__device__ __forceinline__ StreamBlock<OutT> operator()(int p) const {
+		const int a = p / (np.y * np.z), b = (p / np.z) % np.y, c = p % np.z;
+		StreamBlock<OutT> k;
+		k.out = out;
+		k.sy = shape.y;
+		k.sz = shape.z;
+		k.lo0 = a * part.x;
+		k.lo1 = b * part.y;
+		k.lo2 = c * part.z;
+		k.e0 = min(k.lo0 + part.x, shape.x) - k.lo0;
+		k.e1 = min(k.lo1 + part.y, shape.y) - k.lo1;
+		k.e2 = min(k.lo2 + part.z, shape.z) - k.lo2;
+		k.sd = sd;
+		k.sdf = sdf;
+		return k;
+	}
+};
+
+// Many surfaces: one record per stream.
+template <class OutT>
+struct TableBlocks {
+	OutT* arena;
+	const NoiseBlockRec* recs;
+	__device__ __forceinline__ StreamBlock<OutT> operator()(int p) const {
+		const NoiseBlockRec r = recs[p];
+		return StreamBlock<OutT>{arena + r.base, r.sy, r.sz, r.lo0, r.lo1, r.lo2,
+		                         r.e0,           r.e1, r.e2, r.sd,  r.sdf};
+	}
+};
+
 // One thread per partition; partitions in row-major order (stream p fills
 // partition p), noise row-major inside each partition.
-template <class OutT, int SEM>
+template <class OutT, int SEM, class Geo>
 __global__ void noise_partitioned_kernel(const arma_mt_params* __restrict__ params,
-                                         const uint32_t* __restrict__ seeds, int3 shape,
-                                         int3 part, int3 np, double stddev, float stddev_f,
-                                         OutT* __restrict__ out) {
+                                         const uint32_t* __restrict__ seeds, Geo geo,
+                                         int ntotal) {
 	extern __shared__ uint32_t lds_state[];
 	const int tid = threadIdx.x;
 	const int p = blockIdx.x * blockDim.x + tid;
-	const int ntotal = np.x * np.y * np.z;
 	if (p >= ntotal) return;
 	LdsMt g;
 	g.st = lds_state + tid;
 	g.stride = blockDim.x;
 	g.p = params[p];
 	g.seed(seeds ? seeds[p] : 0u);
-	const int a = p / (np.y * np.z), b = (p / np.z) % np.y, c = p % np.z;
-	const int lo0 = a * part.x, lo1 = b * part.y, lo2 = c * part.z;
-	const int e0 = min(lo0 + part.x, shape.x) - lo0;
-	const int e1 = min(lo1 + part.y, shape.y) - lo1;
-	const int e2 = min(lo2 + part.z, shape.z) - lo2;
+	const StreamBlock<OutT> k = geo(p);
 	PolarD nd;
 	PolarF nf;
-	for (int t = 0; t < e0; ++t) {
-		for (int x = 0; x < e1; ++x) {
-			OutT* row = out + ((size_t)(lo0 + t) * shape.y + (lo1 + x)) * shape.z + lo2;
-			for (int y = 0; y < e2; ++y) {
+	for (int t = 0; t < k.e0; ++t) {
+		for (int x = 0; x < k.e1; ++x) {
+			OutT* row = k.out + ((size_t)(k.lo0 + t) * k.sy + (k.lo1 + x)) * k.sz + k.lo2;
+			for (int y = 0; y < k.e2; ++y) {
 				if (SEM == ARMA_NOISE_DOUBLE) {
-					row[y] = OutT(nd.next(g, stddev));
+					row[y] = OutT(nd.next(g, k.sd));
 				} else {
-					row[y] = OutT(nf.next(g, stddev_f));
+					row[y] = OutT(nf.next(g, k.sdf));
 				}
 			}
 		}
@@ -241,10 +284,9 @@ __device__ void twist_parallel(uint32_t* st, const arma_mt_params& p, int tid) {
 	}
 }
 
-template <class OutT, int SEM, int THREADS, int MAXNN>
+template <class OutT, int SEM, int THREADS, int MAXNN, class Geo>
 __global__ __launch_bounds__(THREADS) void noise_stream_kernel(
-	const arma_mt_params* __restrict__ params, const uint32_t* __restrict__ seeds, int3 shape,
-	int3 part, int3 np, double stddev, float stddev_f, OutT* __restrict__ out) {
+	const arma_mt_params* __restrict__ params, const uint32_t* __restrict__ seeds, Geo geo) {
 	constexpr int words_per_attempt = SEM == ARMA_NOISE_DOUBLE ? 4 : 2;
 	constexpr int round_words = THREADS * words_per_attempt;
 	constexpr int KEEP = (MAXNN + THREADS - 1) / THREADS;
@@ -256,12 +298,15 @@ __global__ __launch_bounds__(THREADS) void noise_stream_kernel(
 	const int pid = blockIdx.x;
 	const arma_mt_params p = params[pid];
 	// the block of the grid this stream fills, row-major
-	const int a = pid / (np.y * np.z), b = (pid / np.z) % np.y, c = pid % np.z;
-	const int lo0 = a * part.x, lo1 = b * part.y, lo2 = c * part.z;
-	const int e1 = min(lo1 + part.y, shape.y) - lo1;
-	const int e2 = min(lo2 + part.z, shape.z) - lo2;
-	const long long e12 = (long long)e1 * e2;
-	const long long n = (long long)(min(lo0 + part.x, shape.x) - lo0) * e12;
+	const StreamBlock<OutT> blk = geo(pid);
+	OutT* const out = blk.out;
+	const int sy = blk.sy, sz = blk.sz;
+	const int lo0 = blk.lo0, lo1 = blk.lo1, lo2 = blk.lo2;
+	const int e2 = blk.e2;
+	const double stddev = blk.sd;
+	const float stddev_f = blk.sdf;
+	const long long e12 = (long long)blk.e1 * e2;
+	const long long n = (long long)blk.e0 * e12;
 	// position -> (t, x, y) inside the block; 32-bit quotients through a
 	// double reciprocal (exact after one correction) while positions fit
 	const double r12 = 1.0 / double(e12), r2 = 1.0 / double(e2);
@@ -287,7 +332,7 @@ __global__ __launch_bounds__(THREADS) void noise_stream_kernel(
 			x = int(r / e2);
 			y = int(r - (long long)x * e2);
 		}
-		out[(size_t(lo0 + t) * shape.y + (lo1 + x)) * shape.z + (lo2 + y)] = v;
+		out[(size_t(lo0 + t) * sy + (lo1 + x)) * sz + (lo2 + y)] = v;
 	};
 	if (tid == 0) {
 		uint32_t s = seeds ? seeds[pid] : 0u;
@@ -385,10 +430,10 @@ __device__ __forceinline__ uint32_t twist_word_old(const uint32_t* st, int k,
 	return mt_twist_word(st[k], st[k + 1], st[k + p.mm], p);
 }
 
-template <class OutT, int SEM>
+template <class OutT, int SEM, class Geo>
 __global__ __launch_bounds__(64) void noise_seg_kernel(
-	const arma_mt_params* __restrict__ params, const uint32_t* __restrict__ seeds, int3 shape,
-	int3 part, int3 np, double stddev, float stddev_f, OutT* __restrict__ out, int nstreams) {
+	const arma_mt_params* __restrict__ params, const uint32_t* __restrict__ seeds, Geo geo,
+	int nstreams) {
 	constexpr int WPA = SEM == ARMA_NOISE_DOUBLE ? 4 : 2;  // words per attempt
 	constexpr int RW = kSegLanes * WPA;                     // words per round
 	__shared__ uint32_t st_all[2][32];
@@ -402,12 +447,15 @@ __global__ __launch_bounds__(64) void noise_seg_kernel(
 	arma_mt_params p = params[valid ? pid : 0];
 	const int nn = p.nn;
 	// the block of the grid this stream fills, row-major
-	const int a = pid / (np.y * np.z), b = (pid / np.z) % np.y, c = pid % np.z;
-	const int lo0 = a * part.x, lo1 = b * part.y, lo2 = c * part.z;
-	const int e1 = valid ? min(lo1 + part.y, shape.y) - lo1 : 1;
-	const int e2 = valid ? min(lo2 + part.z, shape.z) - lo2 : 1;
-	const long long e12 = (long long)e1 * e2;
-	const long long n = valid ? (long long)(min(lo0 + part.x, shape.x) - lo0) * e12 : 0;
+	const StreamBlock<OutT> blk = geo(valid ? pid : 0);
+	OutT* const out = blk.out;
+	const int sy = blk.sy, sz = blk.sz;
+	const int lo0 = blk.lo0, lo1 = blk.lo1, lo2 = blk.lo2;
+	const int e2 = blk.e2;
+	const double stddev = blk.sd;
+	const float stddev_f = blk.sdf;
+	const long long e12 = (long long)blk.e1 * e2;
+	const long long n = valid ? (long long)blk.e0 * e12 : 0;
 	const double r12 = 1.0 / double(e12), r2 = 1.0 / double(e2);
 	auto put = [&](long long q, OutT v) {
 		// q < 2^31: 32-bit quotients through a double reciprocal, corrected
@@ -419,7 +467,7 @@ __global__ __launch_bounds__(64) void noise_seg_kernel(
 		uint32_t qx = uint32_t(double(rr) * r2);
 		if (uint64_t(qx) * u2 > rr) --qx;
 		else if (uint64_t(qx + 1) * u2 <= rr) ++qx;
-		out[(size_t(lo0 + qt) * shape.y + (lo1 + qx)) * shape.z + (lo2 + (rr - qx * u2))] = v;
+		out[(size_t(lo0 + qt) * sy + (lo1 + qx)) * sz + (lo2 + (rr - qx * u2))] = v;
 	};
 	if (lane == 0) {
 		uint32_t sd = valid && seeds ? seeds[pid] : 0u;
@@ -528,56 +576,109 @@ void validate_params(const arma_mt_params* mts, int n) {
 	}
 }
 
+// Launches the noise of nstreams streams whose output blocks geo describes,
+// with the kernel their parameter sets allow: a whole workgroup for a single
+// stream, two streams per wave for small dcmt states, one wave per stream up
+// to 64 state words, one workgroup per stream for a few large states, else
+// one thread per stream.
+template <class OutT, int SEM, class Geo>
+void launch_streams(arma_hip_ctx* ctx, const arma_mt_params* hparams,
+                    const arma_mt_params* dparams, const uint32_t* dseeds, Geo geo, int nstreams,
+                    bool small_blocks) {
+	int nn_max = 0;
+	for (int i = 0; i < nstreams; ++i) nn_max = std::max(nn_max, int(hparams[i].nn));
+	if (nstreams == 1) {
+		// the whole grid from one stream (framework=none)
+		noise_stream_kernel<OutT, SEM, 256, kMaxNN, Geo>
+			<<<1, 256, 0, ctx->stream>>>(dparams, dseeds, geo);
+	} else if (small_blocks && seg_kernel_fits(hparams, nstreams)) {
+		noise_seg_kernel<OutT, SEM, Geo>
+			<<<div_ceil(nstreams, 2), 64, 0, ctx->stream>>>(dparams, dseeds, geo, nstreams);
+	} else if (nn_max <= 64) {
+		noise_stream_kernel<OutT, SEM, 64, 64, Geo>
+			<<<nstreams, 64, 0, ctx->stream>>>(dparams, dseeds, geo);
+	} else if (nstreams <= 4 * std::max(ctx->num_cus, 1)) {
+		noise_stream_kernel<OutT, SEM, 256, kMaxNN, Geo>
+			<<<nstreams, 256, 0, ctx->stream>>>(dparams, dseeds, geo);
+	} else {
+		const int tpb = std::min(threads_for_state(nn_max), 64);
+		noise_partitioned_kernel<OutT, SEM, Geo>
+			<<<div_ceil(nstreams, tpb), tpb, size_t(tpb) * nn_max * 4, ctx->stream>>>(
+				dparams, dseeds, geo, nstreams);
+	}
+	ARMA_HIP_CHECK(hipGetLastError());
+}
+
+template <class Geo2>
+void launch_by_type(arma_hip_ctx* ctx, bool fp32, int sem, const arma_mt_params* hparams,
+                    const arma_mt_params* dparams, const uint32_t* dseeds, void* out,
+                    const Geo2& g, int nstreams, bool small_blocks) {
+	if (fp32) {
+		auto geo = g.template make<float>((float*)out);
+		if (sem == ARMA_NOISE_DOUBLE)
+			launch_streams<float, ARMA_NOISE_DOUBLE>(ctx, hparams, dparams, dseeds, geo, nstreams,
+			                                         small_blocks);
+		else
+			launch_streams<float, ARMA_NOISE_FLOAT>(ctx, hparams, dparams, dseeds, geo, nstreams,
+			                                        small_blocks);
+	} else {
+		auto geo = g.template make<double>((double*)out);
+		if (sem == ARMA_NOISE_DOUBLE)
+			launch_streams<double, ARMA_NOISE_DOUBLE>(ctx, hparams, dparams, dseeds, geo,
+			                                          nstreams, small_blocks);
+		else
+			launch_streams<double, ARMA_NOISE_FLOAT>(ctx, hparams, dparams, dseeds, geo,
+			                                         nstreams, small_blocks);
+	}
+}
+
+struct GridSpec {
+	int3 shape, part, np;
+	double sd;
+	float sdf;
+	template <class OutT>
+	GridBlocks<OutT> make(OutT* out) const {
+		return GridBlocks<OutT>{out, shape, part, np, sd, sdf};
+	}
+};
+
+struct TableSpec {
+	const NoiseBlockRec* recs;
+	template <class OutT>
+	TableBlocks<OutT> make(OutT* arena) const {
+		return TableBlocks<OutT>{arena, recs};
+	}
+};
+
 }  // namespace
 
 void launch_white_noise(arma_hip_ctx* ctx, const arma_ar_desc* d, void* eps) {
 	ARMA_HIP_REQUIRE(d->varwn >= 0.0, ARMA_HIP_EINVAL, "variance is less than zero");
 	ARMA_HIP_REQUIRE(d->shape[0] > 0 && d->shape[1] > 0 && d->shape[2] > 0, ARMA_HIP_EINVAL,
 	                 "bad output grid shape");
-	const double stddev = std::sqrt(d->varwn);
-	const float stddev_f = std::sqrt(float(d->varwn));
 	const bool fp32 = d->precision == ARMA_FP32;
 	const int sem = d->noise_semantics;
 	ARMA_HIP_REQUIRE(sem == ARMA_NOISE_DOUBLE || sem == ARMA_NOISE_FLOAT, ARMA_HIP_EINVAL,
 	                 "bad noise semantics");
+	GridSpec g;
+	g.shape = int3{d->shape[0], d->shape[1], d->shape[2]};
+	g.sd = std::sqrt(d->varwn);
+	g.sdf = std::sqrt(float(d->varwn));
 	if (d->noise_mode == ARMA_NOISE_SEQUENTIAL) {
-		ARMA_HIP_REQUIRE(d->n_mts >= 1 && d->mts, ARMA_HIP_EPRNG, "bad number of MT configs");
-		validate_params(d->mts, 1);
-		auto* dp = (arma_mt_params*)ctx->params.get(sizeof(arma_mt_params));
-		upload(ctx, dp, d->mts, sizeof(arma_mt_params));
-		uint32_t* ds = nullptr;
-		if (d->seeds) {
-			ds = (uint32_t*)ctx->seeds.get(sizeof(uint32_t));
-			upload(ctx, ds, d->seeds, sizeof(uint32_t));
-		}
-		const int3 shape{d->shape[0], d->shape[1], d->shape[2]};
-		const int3 one{1, 1, 1};
-		KernelTimer timer(ctx, "noise_sequential");
-#define ARMA_LAUNCH_SEQ(OT, S)                                                                   \
-	noise_stream_kernel<OT, S, 256, kMaxNN><<<1, 256, 0, ctx->stream>>>(dp, ds, shape, shape, one, \
-	                                                                   stddev, stddev_f, (OT*)eps)
-		if (fp32) {
-			if (sem == ARMA_NOISE_DOUBLE) ARMA_LAUNCH_SEQ(float, ARMA_NOISE_DOUBLE);
-			else ARMA_LAUNCH_SEQ(float, ARMA_NOISE_FLOAT);
-		} else {
-			if (sem == ARMA_NOISE_DOUBLE) ARMA_LAUNCH_SEQ(double, ARMA_NOISE_DOUBLE);
-			else ARMA_LAUNCH_SEQ(double, ARMA_NOISE_FLOAT);
-		}
-#undef ARMA_LAUNCH_SEQ
-		ARMA_HIP_CHECK(hipGetLastError());
-		return;
+		g.part = g.shape;
+		g.np = int3{1, 1, 1};
+	} else {
+		ARMA_HIP_REQUIRE(d->noise_mode == ARMA_NOISE_PARTITIONED, ARMA_HIP_EINVAL,
+		                 "bad noise mode");
+		ARMA_HIP_REQUIRE(d->partition[0] > 0 && d->partition[1] > 0 && d->partition[2] > 0,
+		                 ARMA_HIP_EINVAL, "bad partition shape");
+		g.part = int3{d->partition[0], d->partition[1], d->partition[2]};
+		g.np = int3{div_ceil(g.shape.x, g.part.x), div_ceil(g.shape.y, g.part.y),
+		            div_ceil(g.shape.z, g.part.z)};
 	}
-	ARMA_HIP_REQUIRE(d->noise_mode == ARMA_NOISE_PARTITIONED, ARMA_HIP_EINVAL, "bad noise mode");
-	ARMA_HIP_REQUIRE(d->partition[0] > 0 && d->partition[1] > 0 && d->partition[2] > 0,
-	                 ARMA_HIP_EINVAL, "bad partition shape");
-	const int3 shape{d->shape[0], d->shape[1], d->shape[2]};
-	const int3 part{d->partition[0], d->partition[1], d->partition[2]};
-	const int3 np{div_ceil(shape.x, part.x), div_ceil(shape.y, part.y), div_ceil(shape.z, part.z)};
-	const int ntotal = np.x * np.y * np.z;
+	const int ntotal = g.np.x * g.np.y * g.np.z;
 	ARMA_HIP_REQUIRE(d->n_mts >= ntotal && d->mts, ARMA_HIP_EPRNG, "bad number of MT configs");
 	validate_params(d->mts, ntotal);
-	int nn_max = 0;
-	for (int i = 0; i < ntotal; ++i) nn_max = std::max(nn_max, int(d->mts[i].nn));
 	auto* dparams = (arma_mt_params*)ctx->params.get(sizeof(arma_mt_params) * ntotal);
 	upload(ctx, dparams, d->mts, sizeof(arma_mt_params) * ntotal);
 	uint32_t* dseeds = nullptr;
@@ -585,59 +686,11 @@ void launch_white_noise(arma_hip_ctx* ctx, const arma_ar_desc* d, void* eps) {
 		dseeds = (uint32_t*)ctx->seeds.get(sizeof(uint32_t) * ntotal);
 		upload(ctx, dseeds, d->seeds, sizeof(uint32_t) * ntotal);
 	}
-	KernelTimer timer(ctx, "noise_partitioned");
 	// every stream's block must fit 32-bit positions for the fast kernels
-	const bool small_blocks = (long long)part.x * part.y * part.z < (1ll << 31);
-	if (small_blocks && seg_kernel_fits(d->mts, ntotal)) {
-		// two streams per wave
-#define ARMA_LAUNCH_SEG(OT, S)                                                                   \
-	noise_seg_kernel<OT, S><<<div_ceil(ntotal, 2), 64, 0, ctx->stream>>>(                        \
-		dparams, dseeds, shape, part, np, stddev, stddev_f, (OT*)eps, ntotal)
-		if (fp32) {
-			if (sem == ARMA_NOISE_DOUBLE) ARMA_LAUNCH_SEG(float, ARMA_NOISE_DOUBLE);
-			else ARMA_LAUNCH_SEG(float, ARMA_NOISE_FLOAT);
-		} else {
-			if (sem == ARMA_NOISE_DOUBLE) ARMA_LAUNCH_SEG(double, ARMA_NOISE_DOUBLE);
-			else ARMA_LAUNCH_SEG(double, ARMA_NOISE_FLOAT);
-		}
-#undef ARMA_LAUNCH_SEG
-		ARMA_HIP_CHECK(hipGetLastError());
-		return;
-	}
-	if (nn_max <= 64) {
-		// one wave per stream (dcmt exponents up to 1279)
-#define ARMA_LAUNCH_WAVE(OT, S)                                                                  \
-	noise_stream_kernel<OT, S, 64, 64><<<ntotal, 64, 0, ctx->stream>>>(dparams, dseeds, shape,   \
-	                                                                  part, np, stddev,         \
-	                                                                  stddev_f, (OT*)eps)
-		if (fp32) {
-			if (sem == ARMA_NOISE_DOUBLE) ARMA_LAUNCH_WAVE(float, ARMA_NOISE_DOUBLE);
-			else ARMA_LAUNCH_WAVE(float, ARMA_NOISE_FLOAT);
-		} else {
-			if (sem == ARMA_NOISE_DOUBLE) ARMA_LAUNCH_WAVE(double, ARMA_NOISE_DOUBLE);
-			else ARMA_LAUNCH_WAVE(double, ARMA_NOISE_FLOAT);
-		}
-#undef ARMA_LAUNCH_WAVE
-		ARMA_HIP_CHECK(hipGetLastError());
-		return;
-	}
-	// large states: one thread per stream with the state in LDS
-	const int tpb = std::min(threads_for_state(nn_max), 64);
-	const int blocks = div_ceil(ntotal, tpb);
-	const size_t lds = size_t(tpb) * nn_max * 4;
-#define ARMA_LAUNCH_PART(OT, S)                                                                  \
-	noise_partitioned_kernel<OT, S><<<blocks, tpb, lds, ctx->stream>>>(dparams, dseeds, shape,   \
-	                                                                  part, np, stddev,         \
-	                                                                  stddev_f, (OT*)eps)
-	if (fp32) {
-		if (sem == ARMA_NOISE_DOUBLE) ARMA_LAUNCH_PART(float, ARMA_NOISE_DOUBLE);
-		else ARMA_LAUNCH_PART(float, ARMA_NOISE_FLOAT);
-	} else {
-		if (sem == ARMA_NOISE_DOUBLE) ARMA_LAUNCH_PART(double, ARMA_NOISE_DOUBLE);
-		else ARMA_LAUNCH_PART(double, ARMA_NOISE_FLOAT);
-	}
-#undef ARMA_LAUNCH_PART
-	ARMA_HIP_CHECK(hipGetLastError());
+	const bool small_blocks = (long long)g.part.x * g.part.y * g.part.z < (1ll << 31);
+	KernelTimer timer(ctx, d->noise_mode == ARMA_NOISE_SEQUENTIAL ? "noise_sequential"
+	                                                               : "noise_partitioned");
+	launch_by_type(ctx, fp32, sem, d->mts, dparams, dseeds, eps, g, ntotal, small_blocks);
 }
 
 void launch_mt_raw(arma_hip_ctx* ctx, const arma_mt_params* mts, const uint32_t* seeds,
@@ -658,5 +711,102 @@ void launch_mt_raw(arma_hip_ctx* ctx, const arma_mt_params* mts, const uint32_t*
 		dparams, dseeds, n_streams, draws, out);
 	ARMA_HIP_CHECK(hipGetLastError());
 }
+
+struct NoiseBatch {
+	int precision = 0;
+	int sem = ARMA_NOISE_DOUBLE;
+	int nstreams = 0;
+	bool small_blocks = true;
+	std::vector<arma_mt_params> hparams;
+	Scratch params, seeds, recs;
+};
+
+NoiseBatch* noise_batch_create(arma_hip_ctx* ctx, int precision,
+                               const std::vector<BatchMember>& m) {
+	auto* b = new NoiseBatch;
+	try {
+		noise_batch_build(ctx, b, precision, m);
+	} catch (...) {
+		delete b;
+		throw;
+	}
+	return b;
+}
+
+NoiseBatch* noise_batch_new() { return new NoiseBatch; }
+
+void noise_batch_build(arma_hip_ctx* ctx, NoiseBatch* b, int precision,
+                       const std::vector<BatchMember>& m) {
+	{
+		b->precision = precision;
+		b->small_blocks = true;
+		b->hparams.clear();
+		ARMA_HIP_REQUIRE(!m.empty(), ARMA_HIP_EINVAL, "empty batch");
+		b->sem = m[0].d->noise_semantics;
+		std::vector<uint32_t> seeds;
+		std::vector<NoiseBlockRec> recs;
+		for (const BatchMember& x : m) {
+			const arma_ar_desc* d = x.d;
+			ARMA_HIP_REQUIRE(d->varwn >= 0.0, ARMA_HIP_EINVAL, "variance is less than zero");
+			ARMA_HIP_REQUIRE(d->noise_semantics == b->sem, ARMA_HIP_EINVAL,
+			                 "every surface of a batch needs the same noise semantics");
+			// the generated block: slices t0 .. nt-1
+			const int gt = d->shape[0] - x.t0;
+			ARMA_HIP_REQUIRE(x.t0 >= 0 && gt > 0, ARMA_HIP_EINVAL, "bad history length");
+			int3 part{gt, d->shape[1], d->shape[2]};
+			if (d->noise_mode == ARMA_NOISE_PARTITIONED) {
+				ARMA_HIP_REQUIRE(d->partition[0] > 0 && d->partition[1] > 0 && d->partition[2] > 0,
+				                 ARMA_HIP_EINVAL, "bad partition shape");
+				part = int3{d->partition[0], d->partition[1], d->partition[2]};
+			} else {
+				ARMA_HIP_REQUIRE(d->noise_mode == ARMA_NOISE_SEQUENTIAL, ARMA_HIP_EINVAL,
+				                 "bad noise mode");
+			}
+			const int np0 = div_ceil(gt, part.x), np1 = div_ceil(d->shape[1], part.y),
+			          np2 = div_ceil(d->shape[2], part.z);
+			const int n = np0 * np1 * np2;
+			ARMA_HIP_REQUIRE(d->n_mts >= n && d->mts, ARMA_HIP_EPRNG, "bad number of MT configs");
+			validate_params(d->mts, n);
+			b->small_blocks = b->small_blocks && (long long)part.x * part.y * part.z < (1ll << 31);
+			const double sd = std::sqrt(d->varwn);
+			const float sdf = std::sqrt(float(d->varwn));
+			for (int p = 0; p < n; ++p) {
+				const int a = p / (np1 * np2), bb = (p / np2) % np1, c = p % np2;
+				NoiseBlockRec r;
+				r.base = (long long)(x.zoff + size_t(x.t0) * d->shape[1] * d->shape[2]);
+				r.sy = d->shape[1];
+				r.sz = d->shape[2];
+				r.lo0 = a * part.x;
+				r.lo1 = bb * part.y;
+				r.lo2 = c * part.z;
+				r.e0 = std::min(r.lo0 + part.x, gt) - r.lo0;
+				r.e1 = std::min(r.lo1 + part.y, d->shape[1]) - r.lo1;
+				r.e2 = std::min(r.lo2 + part.z, d->shape[2]) - r.lo2;
+				r.sd = sd;
+				r.sdf = sdf;
+				recs.push_back(r);
+				b->hparams.push_back(d->mts[p]);
+				seeds.push_back(d->seeds ? d->seeds[p] : 0u);
+			}
+		}
+		b->nstreams = int(recs.size());
+		b->params.get(sizeof(arma_mt_params) * recs.size());
+		b->seeds.get(sizeof(uint32_t) * recs.size());
+		b->recs.get(sizeof(NoiseBlockRec) * recs.size());
+		upload(ctx, b->params.ptr, b->hparams.data(), sizeof(arma_mt_params) * recs.size());
+		upload(ctx, b->seeds.ptr, seeds.data(), sizeof(uint32_t) * recs.size());
+		upload(ctx, b->recs.ptr, recs.data(), sizeof(NoiseBlockRec) * recs.size());
+	}
+}
+
+void noise_batch_launch(arma_hip_ctx* ctx, NoiseBatch* b, void* arena) {
+	KernelTimer timer(ctx, "noise_batch");
+	TableSpec g{(const NoiseBlockRec*)b->recs.ptr};
+	launch_by_type(ctx, b->precision == ARMA_FP32, b->sem, b->hparams.data(),
+	               (const arma_mt_params*)b->params.ptr, (const uint32_t*)b->seeds.ptr, arena, g,
+	               b->nstreams, b->small_blocks);
+}
+
+void noise_batch_destroy(NoiseBatch* b) { delete b; }
 
 }  // namespace arma_hip

@@ -49,6 +49,13 @@ WORKLOADS = {
         acf=("standing", 0.8, (0.8, 0.8)), shape=(2048, 512, 512), order=7,
         partition=(32, 64, 64), seed=1, depth=12.0, lbound=(0.0, -10.0),
         ubound=(2047.0, 0.0), npoints=(2048, 32), precision="fp32"),
+    # many uneven surfaces in one persistent launch: 64 surfaces, seed 42,
+    # t ~ U[64,1024], x,y ~ U[32,384], order per surface from {3,5,7,9},
+    # propagating ACF with alpha ~ U[0.04,0.1], beta ~ U[0.5,1.2]; FP32,
+    # autotuned tile shape / schedule / residency, launches replayed as a graph
+    "batch-64-fp32": dict(kind="batch", count=64, seed=42, t=(64, 1024), xy=(32, 384),
+                          orders=(3, 5, 7, 9), alpha=(0.04, 0.1), beta=(0.5, 1.2),
+                          partition=(64, 64, 64), precision="fp32"),
     # CPU-runnable plain AR case (no potential)
     "ar-standing-128x64x64-fp64": dict(
         acf=("standing", 0.8, (0.8, 0.8)), shape=(128, 64, 64), order=7,
@@ -69,6 +76,128 @@ def setup_model(arma, w):
     phi, _ = arma._abi.yule_walker_choi(acf, w["order"])
     varwn = arma._abi.white_noise_variance(acf, phi)
     return acf, phi, varwn
+
+
+def batch_members(arma, w):
+    """The surfaces of a batch workload, drawn from a seeded generator.  A
+    draw whose Choi coefficients give a negative white-noise variance (the
+    reference rejects it with std::invalid_argument) is redrawn."""
+    import numpy as np
+    rng = np.random.default_rng(w["seed"])
+    out = []
+    for _ in range(w["count"]):
+        nt = int(rng.integers(w["t"][0], w["t"][1] + 1))
+        nx = int(rng.integers(w["xy"][0], w["xy"][1] + 1))
+        ny = int(rng.integers(w["xy"][0], w["xy"][1] + 1))
+        order = int(rng.choice(w["orders"]))
+        for _ in range(1000):
+            a = float(rng.uniform(*w["alpha"]))
+            b = float(rng.uniform(*w["beta"]))
+            acf = arma._abi.acf(arma._abi.ACF_PROPAGATING_WAVE, (10, 10, 10), velocity=b,
+                                alpha=(a, a, a), beta=(-b, 0.0))
+            phi, _ = arma._abi.yule_walker_choi(acf, order)
+            varwn = arma._abi.white_noise_variance(acf, phi)
+            if varwn > 0:
+                break
+        out.append(dict(shape=(nt, nx, ny), order=order, alpha=a, beta=b, phi=phi,
+                        varwn=varwn, gamma0=float(acf[0, 0, 0])))
+    return out
+
+
+def run_batch(args, w):
+    """Batching workload: noise, recurrence and rescale of every surface,
+    captured once as a graph and replayed per step.  value = surface points
+    per second.  Per-kernel times come from an eager pass of the same work
+    with per-kernel events, right after the timed replays."""
+    import numpy as np
+    import torch
+    import arma_amd
+    from arma_amd import _abi
+    t0 = time.time()
+    mem = batch_members(arma_amd, w)
+    part = w["partition"]
+    nstreams = [math.prod(math.ceil(m["shape"][d] / part[d]) for d in range(3)) for m in mem]
+    params = _abi.dcmt_p521_params(max(nstreams))
+    log(f"bench: {len(mem)} surfaces set up in {time.time() - t0:.1f}s, "
+        f"{sum(nstreams)} noise streams")
+    ctx = arma_amd.Context(0, torch_stream=False)
+    prec = arma_amd.FP32 if w["precision"] == "fp32" else arma_amd.FP64
+    tdt = torch.float32 if prec == arma_amd.FP32 else torch.float64
+    descs = [ctx.ar_desc(m["shape"], m["phi"], m["varwn"], params,
+                         seeds=np.full(nstreams[i], w["seed"] + i, np.uint32),
+                         noise_mode=arma_amd.NOISE_PARTITIONED, partition=part, precision=prec,
+                         algorithm=arma_amd.AR_WAVEFRONT) for i, m in enumerate(mem)]
+    batch = ctx.batch(descs, [m["gamma0"] for m in mem], prec)
+    arena = torch.empty(batch.arena_elements, dtype=tdt, device="cuda:0")
+    torch.cuda.synchronize()
+    t0 = time.time()
+    trials = batch.tune(arena, reps=1)
+    cfg, grid = batch.config()
+    log(f"bench: tuned in {time.time() - t0:.1f}s over {len(trials)} candidates: "
+        f"tile_x {cfg[0]}, schedule {cfg[1]}, slots/CU {cfg[2]}, grid {grid}")
+    batch.capture(arena)
+    for i in range(args.warmup):
+        batch.replay()
+        ctx.synchronize()
+        log(f"bench: warmup step {i + 1}/{args.warmup} done")
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        batch.replay()
+        ctx.synchronize()
+        log(f"bench: step {i + 1}/{args.steps} done")
+    t_total = time.perf_counter() - t_start
+    # eager pass with per-kernel events
+    _abi.check(_abi.lib().arma_hip_profile(ctx.h, 1))
+    parse_profile(ctx)
+    t_e = time.perf_counter()
+    for i in range(args.steps):
+        batch.generate(arena)
+        ctx.synchronize()
+    t_eager = time.perf_counter() - t_e
+    prof = parse_profile(ctx)
+    _abi.check(_abi.lib().arma_hip_profile(ctx.h, 0))
+    pts = sum(math.prod(m["shape"]) for m in mem)
+    flops = sum(2.0 * m["phi"].size * math.prod(m["shape"]) for m in mem)
+    esize = 4 if prec == arma_amd.FP32 else 8
+    kernels = {}
+    for name, (us, cnt) in prof.items():
+        mean_s = us / cnt * 1e-6
+        e = {"mean_us": us / cnt, "launches": cnt, "share": us / 1e6 / t_eager}
+        if name == "ar_batch":
+            e.update(bound="mfma", unit="TFLOP/s", achieved=flops / mean_s / 1e12,
+                     peak=FP32_PEAK_TFLOPS if prec == arma_amd.FP32 else FP64_PEAK_TFLOPS)
+        elif name == "noise_batch":
+            e.update(bound="hbm", unit="GB/s", achieved=pts * esize / mean_s / 1e9,
+                     peak=HBM_PEAK_GBS)
+        kernels[name] = e
+    dom = max(kernels, key=lambda k: kernels[k]["share"])
+    k = kernels[dom]
+    roofline = None
+    if "achieved" in k:
+        roofline = {"bound": k["bound"], "achieved": round(k["achieved"], 3), "peak": k["peak"],
+                    "unit": k["unit"], "frac": round(k["achieved"] / k["peak"], 4),
+                    "traffic": None, "kernel": dom}
+    ms = 1000.0 * t_total / args.steps
+    line = {
+        "metric": "surface pts/s + potential pts/s (FP32 vs FP64) at RMS<=1e-5; p50/p99 latency",
+        "value": pts / (t_total / args.steps), "unit": "pts/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if prec == arma_amd.FP32 else "f64", "data": "synthetic",
+        "config": {"workload": args.workload, "surfaces": len(mem), "points": pts,
+                   "orders": {str(o): sum(1 for m in mem if m["order"] == o)
+                              for o in w["orders"]},
+                   "noise_streams": sum(nstreams), "tile_x": cfg[0], "schedule": cfg[1],
+                   "slots_per_cu": cfg[2], "grid": grid, "graph": True,
+                   "parallelism": "single-gpu"},
+        "eager_ms_per_step": 1000.0 * t_eager / args.steps,
+        "tuning": [{"tile_x": c[0], "schedule": c[1], "slots_per_cu": c[2], "grid": g,
+                    "ar_ms": round(t, 3)} for c, g, t in trials],
+        "roofline": roofline, "kernels": kernels, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    batch.close()
+    ctx.close()
 
 
 def sample_shape(w):
@@ -178,6 +307,9 @@ def main():
     from arma_amd import _abi
 
     w = WORKLOADS[args.workload]
+    if w.get("kind") == "batch":
+        run_batch(args, w)
+        return
     fp32 = w["precision"] == "fp32"
     prec = arma_amd.FP32 if fp32 else arma_amd.FP64
     tdt = torch.float32 if fp32 else torch.float64

@@ -206,6 +206,160 @@ double arma_hip_white_noise_variance(const double* acf, const int32_t acf_shape[
 void arma_hip_partition_shape(const int32_t partition[3], const int32_t grid[3],
                               const int32_t order[3], int32_t parallelism, int32_t out[3]);
 
+/* ---- Batching: many surfaces in one persistent launch ----
+ *
+ * No counterpart in the reference, whose driver makes one surface per run
+ * (src/arma_driver.cc:29-87).  Every member is generated with exactly the
+ * semantics of arma_hip_ar_generate followed by arma_hip_rescale on its own
+ * descriptor, and its values do not depend on the rest of the batch, the
+ * tile shape or the schedule.  A batch owns its device tables and scratch;
+ * generate, capture and replay only enqueue kernels on the context stream. */
+
+enum arma_batch_schedule {
+	ARMA_BATCH_AUTO = 0,
+	ARMA_BATCH_SURFACE_MAJOR = 1,  /* surfaces by decreasing work, tile rows in order */
+	ARMA_BATCH_INTERLEAVED = 2     /* tile rows dealt round-robin over the surfaces */
+};
+
+typedef struct arma_batch_config {
+	int32_t tile_x;        /* 8, 16 or 32: tiles of tile_x x (256 / tile_x) columns; 0 = auto */
+	int32_t schedule;      /* enum arma_batch_schedule */
+	int32_t slots_per_cu;  /* resident workgroups per CU; 0 = as many as fit */
+} arma_batch_config;
+
+typedef struct arma_batch_desc {
+	int32_t n_surfaces;
+	const arma_ar_desc* surfaces;  /* one generator per surface (algorithm, tile ignored) */
+	const double* gamma0;          /* per-surface rescale target acf(0,0,0); NULL or <= 0 = none */
+	const int32_t* history;        /* per-surface count of leading slices that the caller
+	                                  writes into the arena before generate and that enter
+	                                  the recurrence as given values (a realtime stream's
+	                                  last slices); noise, partition and output cover
+	                                  slices history..nt-1 only.  NULL = none */
+	int32_t precision;             /* enum arma_precision, shared by all surfaces */
+	arma_batch_config config;      /* all zero: the tuned configuration of an identical
+	                                  batch on this context, else 16 x 16 surface-major */
+} arma_batch_desc;
+
+typedef struct arma_batch_trial {
+	arma_batch_config config;
+	int32_t grid;                  /* workgroups launched */
+	double ms;                     /* best AR launch time over the repetitions */
+} arma_batch_trial;
+
+typedef struct arma_hip_batch arma_hip_batch;
+
+int arma_hip_batch_create(arma_hip_ctx* ctx, const arma_batch_desc* desc, arma_hip_batch** out);
+void arma_hip_batch_destroy(arma_hip_batch* batch);
+/* offsets[n_surfaces + 1]: the first arena element of every surface, then
+   the arena length in elements. */
+int arma_hip_batch_layout(const arma_hip_batch* batch, int64_t* offsets);
+/* Noise, recurrence and rescale of every surface into arena (device). */
+int arma_hip_batch_generate(arma_hip_ctx* ctx, arma_hip_batch* batch, void* arena);
+/* Records the launch sequence of generate for arena into a HIP graph held
+   by the batch; replay launches it on the context stream. */
+int arma_hip_batch_capture(arma_hip_ctx* ctx, arma_hip_batch* batch, void* arena);
+int arma_hip_batch_replay(arma_hip_ctx* ctx, arma_hip_batch* batch);
+/* Autotuner: times the recurrence launch of every candidate tile shape,
+   schedule and residency on this batch (noise regenerated before each
+   timing, reps timings each), keeps the fastest, and remembers it on the
+   context for identical batches.  trials (optional) receives up to
+   max_trials timings; n_trials the number of candidates timed. */
+int arma_hip_batch_tune(arma_hip_ctx* ctx, arma_hip_batch* batch, void* arena, int32_t reps,
+                        arma_batch_trial* trials, int32_t max_trials, int32_t* n_trials);
+int arma_hip_batch_get_config(const arma_hip_batch* batch, arma_batch_config* config,
+                              int32_t* grid);
+int arma_hip_batch_set_config(arma_hip_ctx* ctx, arma_hip_batch* batch,
+                              const arma_batch_config* config);
+/* Post-rescale factors of the last generate or replay (0 where none). */
+int arma_hip_batch_factors(arma_hip_ctx* ctx, arma_hip_batch* batch, double* out);
+
+/* ---- Serving: continuous batching of surface and potential requests ----
+ *
+ * No reference counterpart (the reference makes one surface per run,
+ * src/arma_driver.cc:29-87).  A server owns `lanes` contexts, each with a
+ * HIP stream and a host thread.  Requests queue as they arrive; a scheduler
+ * admits them in arrival order while their device bytes (outputs plus the
+ * working set while in flight) fit hbm_budget, and gives each admitted
+ * group to a free lane as one batch (one persistent recurrence launch over
+ * its surfaces, then their potentials), so work from different lanes
+ * overlaps on the device.  Every result equals the one the single-surface
+ * entry points give for the same descriptor.
+ *
+ * A realtime stream (open_stream) keeps its last phi_shape[0] - 1 slices on
+ * the device; each STREAM_CHUNK request generates its next `chunk` slices.
+ * Chunk k draws its noise with partition (chunk, px, py) and seeds
+ * seeds[0] + k * columns + column, so the chunks concatenate to exactly the
+ * surface that arma_hip_ar_generate makes with partition (chunk, px, py),
+ * mts[p % columns] and seeds seeds[0] + p. */
+
+enum arma_request_kind {
+	ARMA_REQ_SURFACE = 0,       /* noise + recurrence + rescale, optionally the potential */
+	ARMA_REQ_STREAM_CHUNK = 1   /* the next chunk of a realtime stream */
+};
+
+typedef struct arma_server_config {
+	int32_t lanes;              /* batches in flight at once (1..16) */
+	int32_t max_batch;          /* most requests per batch; 0 = 64 */
+	int64_t hbm_budget;         /* device bytes admitted at once */
+	int32_t precision;          /* enum arma_precision of every request */
+	arma_batch_config batch;    /* tile shape / schedule of every batch (0 = defaults) */
+} arma_server_config;
+
+typedef struct arma_request {
+	int32_t kind;               /* enum arma_request_kind */
+	arma_ar_desc surface;       /* SURFACE: the generator (copied at submit) */
+	double gamma0;              /* SURFACE: rescale target acf(0,0,0); <= 0 = none */
+	int32_t want_potential;     /* SURFACE: also the linear potential described by: */
+	arma_pot_desc potential;    /*   (its shape must equal the surface's) */
+	int64_t stream;             /* STREAM_CHUNK: handle from arma_hip_server_open_stream */
+	double arrival;             /* arrival time on arma_hip_server_clock(); 0 = now */
+} arma_request;
+
+typedef struct arma_completion {
+	int64_t id;
+	int32_t status;             /* ARMA_HIP_OK or the error code of its batch */
+	int32_t kind;
+	int64_t stream, chunk;      /* STREAM_CHUNK: its stream and chunk index */
+	const void* zeta;           /* device surface (or chunk), valid until release */
+	const void* phi;            /* device potential (t, z, x, y), or NULL */
+	int32_t shape[3];           /* of zeta */
+	double t_arrival, t_admit, t_done;   /* arma_hip_server_clock() seconds */
+	int32_t batch_size;         /* requests in the batch it ran in */
+	int32_t lane;
+} arma_completion;
+
+typedef struct arma_server_stats {
+	int64_t in_use, peak, budget;       /* admitted device bytes */
+	int64_t batches, batched_requests, completed, pending;
+	int32_t busy_lanes;
+} arma_server_stats;
+
+typedef struct arma_hip_server arma_hip_server;
+
+double arma_hip_server_clock(void);
+int arma_hip_server_create(int device, const arma_server_config* config,
+                           arma_hip_server** out);
+/* Drains the admitted work, drops what is still queued, frees everything. */
+void arma_hip_server_destroy(arma_hip_server* server);
+/* desc: shape[1..2] = (nx, ny), phi, varwn, noise semantics, partition[1..2]
+   = (px, py) when noise_mode is partitioned, mts (one per column of
+   partitions), seeds[0] = base seed; shape[0] is ignored. */
+int arma_hip_server_open_stream(arma_hip_server* server, const arma_ar_desc* desc,
+                                int32_t chunk, int64_t* stream);
+int arma_hip_server_close_stream(arma_hip_server* server, int64_t stream);
+/* ARMA_HIP_ENOMEM when the request alone exceeds the budget. */
+int arma_hip_server_submit(arma_hip_server* server, const arma_request* request, int64_t* id);
+/* Up to max completions, waiting up to wait_ms (-1: until one is there). */
+int arma_hip_server_poll(arma_hip_server* server, arma_completion* out, int32_t max,
+                         int32_t* n, int32_t wait_ms);
+/* Frees a polled request's outputs and returns its bytes to the budget. */
+int arma_hip_server_release(arma_hip_server* server, int64_t id);
+int arma_hip_server_stats(arma_hip_server* server, arma_server_stats* stats);
+/* Copies bytes from a result pointer (device) to dst (host or device);
+   complete on return. */
+int arma_hip_server_copy(arma_hip_server* server, void* dst, const void* src, size_t bytes);
+
 #ifdef __cplusplus
 }
 #endif
