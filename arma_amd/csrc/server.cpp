@@ -42,11 +42,12 @@ double now_s() {
 
 size_t align64(size_t n) { return (n + 63) / 64 * 64; }
 
-// Device memory of a batch's surfaces, freed when the last of them is released.
+// Device memory of a batch's surfaces, freed when the last of them is
+// released.  Only the lane that allocated it touches it (all its requests
+// return to that lane to be freed), so the count needs no lock.
 struct Arena {
 	void* ptr = nullptr;
 	int refs = 0;
-	hipStream_t stream = nullptr;
 };
 
 struct Stream {
@@ -84,7 +85,6 @@ struct Request {
 	Arena* arena = nullptr;
 	size_t zoff = 0;         // element offset of the surface in the arena
 	void* phi_out = nullptr;
-	hipStream_t phi_stream = nullptr;
 	int batch_size = 0, lane = -1;
 };
 
@@ -95,6 +95,7 @@ struct Lane {
 	ArBatch* ar = nullptr;
 	RescaleBatch* rescale = nullptr;
 	std::vector<Request*> job;
+	std::vector<Request*> frees;  // released requests whose memory this lane frees
 	bool busy = false;
 	std::thread th;
 	std::condition_variable cv;
@@ -118,6 +119,10 @@ struct arma_hip_server {
 	std::vector<std::unique_ptr<arma_hip::Lane>> lanes;
 	std::thread scheduler;
 	bool stop = false;
+	// bumped on every event that can make a queued request admissible (a
+	// submission, a finished batch, freed memory); the scheduler sleeps until
+	// it changes
+	uint64_t epoch = 0;
 	int64_t next_id = 1, next_stream = 1;
 	int64_t in_use = 0, peak = 0;
 	int64_t batches = 0, batched_requests = 0, completed = 0;
@@ -150,10 +155,9 @@ void run_job(arma_hip_server* s, Lane* L) {
 	}
 	Arena* arena = new Arena;
 	arena->refs = int(job.size());
-	arena->stream = ctx->stream;
+	for (Request* r : job) r->arena = arena;
 	ARMA_HIP_CHECK(hipMallocAsync(&arena->ptr, total * es, ctx->stream));
 	for (Request* r : job) {
-		r->arena = arena;
 		mem.push_back(BatchMember{r->zoff, &r->desc, r->gamma0, r->history});
 		if (r->history > 0) {
 			const size_t hb = size_t(r->history) * r->desc.shape[1] * r->desc.shape[2] * es;
@@ -171,7 +175,6 @@ void run_job(arma_hip_server* s, Lane* L) {
 		char* z = (char*)arena->ptr + r->zoff * es;
 		if (r->want_potential) {
 			ARMA_HIP_CHECK(hipMallocAsync(&r->phi_out, r->pot_bytes, ctx->stream));
-			r->phi_stream = ctx->stream;
 			launch_linear_potential(ctx, &r->pot, z, r->phi_out, nullptr);
 		}
 		if (r->stream) {
@@ -186,11 +189,42 @@ void run_job(arma_hip_server* s, Lane* L) {
 	ARMA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
 }
 
+// Frees a request's device memory on the stream of the lane that made it
+// (called by that lane's thread, or after every lane thread has ended) and
+// returns the bytes it still held in the budget.
+int64_t free_request(hipStream_t stream, Request* r) {
+	if (r->arena) {
+		if (--r->arena->refs == 0) {
+			if (r->arena->ptr) (void)hipFreeAsync(r->arena->ptr, stream);
+			delete r->arena;
+		}
+		r->arena = nullptr;
+	}
+	if (r->phi_out) (void)hipFreeAsync(r->phi_out, stream);
+	r->phi_out = nullptr;
+	const int64_t held = r->status == ARMA_HIP_OK ? int64_t(r->out_bytes + r->pot_bytes) : 0;
+	delete r;
+	return held;
+}
+
 void lane_main(arma_hip_server* s, Lane* L) {
 	(void)hipSetDevice(s->device);
 	std::unique_lock<std::mutex> lk(s->mu);
 	for (;;) {
-		L->cv.wait(lk, [&] { return s->stop || !L->job.empty(); });
+		L->cv.wait(lk, [&] { return s->stop || !L->job.empty() || !L->frees.empty(); });
+		if (!L->frees.empty()) {
+			// HIP calls only outside the lock, on this lane's own stream
+			std::vector<Request*> frees;
+			frees.swap(L->frees);
+			lk.unlock();
+			int64_t back = 0;
+			for (Request* r : frees) back += free_request(L->ctx->stream, r);
+			lk.lock();
+			s->in_use -= back;
+			++s->epoch;
+			s->cv_sched.notify_all();
+			continue;
+		}
 		if (L->job.empty()) return;  // stop with nothing to do
 		lk.unlock();
 		int status = ARMA_HIP_OK;
@@ -221,6 +255,7 @@ void lane_main(arma_hip_server* s, Lane* L) {
 		}
 		L->job.clear();
 		L->busy = false;
+		++s->epoch;
 		s->cv_done.notify_all();
 		s->cv_sched.notify_all();
 	}
@@ -228,15 +263,12 @@ void lane_main(arma_hip_server* s, Lane* L) {
 
 void scheduler_main(arma_hip_server* s) {
 	std::unique_lock<std::mutex> lk(s->mu);
+	uint64_t seen = ~uint64_t(0);
 	for (;;) {
-		s->cv_sched.wait(lk, [&] {
-			if (s->stop) return true;
-			if (s->pending.empty()) return false;
-			for (auto& L : s->lanes)
-				if (!L->busy) return true;
-			return false;
-		});
+		// sleep (releasing the lock) until something changed
+		s->cv_sched.wait(lk, [&] { return s->stop || s->epoch != seen; });
 		if (s->stop) return;
+		seen = s->epoch;
 		for (auto& L : s->lanes) {
 			if (L->busy || s->pending.empty()) continue;
 			// admit in arrival order while the budget allows; a chunk whose
@@ -270,20 +302,6 @@ void scheduler_main(arma_hip_server* s) {
 			L->cv.notify_one();
 		}
 	}
-}
-
-void free_request(arma_hip_server* s, Request* r) {
-	if (r->arena) {
-		if (--r->arena->refs == 0) {
-			if (r->arena->ptr) (void)hipFreeAsync(r->arena->ptr, r->arena->stream);
-			delete r->arena;
-		}
-		r->arena = nullptr;
-	}
-	if (r->phi_out) (void)hipFreeAsync(r->phi_out, r->phi_stream);
-	r->phi_out = nullptr;
-	if (r->status == ARMA_HIP_OK) s->in_use -= int64_t(r->out_bytes + r->pot_bytes);
-	delete r;
 }
 
 void copy_desc(const arma_ar_desc& src, arma_ar_desc& d, std::vector<double>& phi,
@@ -364,6 +382,14 @@ int arma_hip_server_create(int device, const arma_server_config* cfg, arma_hip_s
 			L->rescale = arma_hip::rescale_batch_new();
 			s->lanes.push_back(std::move(L));
 		}
+		// keep freed batch memory in the device's pool instead of returning
+		// it to the driver at every synchronisation
+		ARMA_HIP_CHECK(hipSetDevice(device));
+		hipMemPool_t pool = nullptr;
+		if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool) {
+			uint64_t keep = uint64_t(cfg->hbm_budget);
+			(void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+		}
 		arma_hip_server* sp = s.get();
 		for (auto& L : s->lanes) L->th = std::thread(arma_hip::lane_main, sp, L.get());
 		s->scheduler = std::thread(arma_hip::scheduler_main, sp);
@@ -388,9 +414,11 @@ int arma_hip_server_open_stream(arma_hip_server* s, const arma_ar_desc* desc, in
 		ARMA_HIP_REQUIRE(st->px > 0 && st->py > 0, ARMA_HIP_EINVAL, "bad partition shape");
 		st->columns = arma_hip::div_ceil(st->nx, st->px) * arma_hip::div_ceil(st->ny, st->py);
 		ARMA_HIP_REQUIRE(desc->n_mts >= st->columns, ARMA_HIP_EPRNG, "bad number of MT configs");
-		std::vector<uint32_t> unused;
-		arma_hip::copy_desc(*desc, st->desc, st->phi, st->mts, unused);
-		st->desc.seeds = nullptr;
+		arma_ar_desc d = *desc;
+		d.seeds = nullptr;  // only seeds[0], the stream's base seed, is read
+		d.n_mts = st->columns;
+		std::vector<uint32_t> none;
+		arma_hip::copy_desc(d, st->desc, st->phi, st->mts, none);
 		st->seed = desc->seeds ? desc->seeds[0] : 0u;
 		const size_t hb = std::max<size_t>(1, size_t(st->hist) * st->nx * st->ny) *
 		                  arma_hip::elem(s->precision);
@@ -447,6 +475,9 @@ int arma_hip_server_submit(arma_hip_server* s, const arma_request* q, int64_t* i
 				                 ARMA_HIP_EINVAL, "bad domain");
 				ARMA_HIP_REQUIRE(!(r->pot.lbound[1] < -r->pot.depth), ARMA_HIP_EINVAL,
 				                 "min(z) is less than -h");
+				auto pow2 = [](int n) { return n >= 4 && n <= 1024 && (n & (n - 1)) == 0; };
+				ARMA_HIP_REQUIRE(pow2(r->pot.shape[1]) && pow2(r->pot.shape[2]), ARMA_HIP_EINVAL,
+				                 "surface extents along x and y must be powers of two in [4, 1024]");
 				r->pot.precision = s->precision;
 				r->pot_bytes = size_t(r->pot.npoints[0]) * r->pot.npoints[1] * r->pot.shape[1] *
 				               r->pot.shape[2] * es;
@@ -489,6 +520,7 @@ int arma_hip_server_submit(arma_hip_server* s, const arma_request* q, int64_t* i
 		r->id = s->next_id++;
 		*id = r->id;
 		s->pending.push_back(r.release());
+		++s->epoch;
 		s->cv_sched.notify_all();
 	});
 }
@@ -539,13 +571,15 @@ int arma_hip_server_poll(arma_hip_server* s, arma_completion* out, int32_t max, 
 int arma_hip_server_release(arma_hip_server* s, int64_t id) {
 	return sguard([&] {
 		ARMA_HIP_REQUIRE(s, ARMA_HIP_EINVAL, "null server");
-		ARMA_HIP_CHECK(hipSetDevice(s->device));
 		std::lock_guard<std::mutex> g(s->mu);
 		auto it = s->finished.find(id);
 		ARMA_HIP_REQUIRE(it != s->finished.end(), ARMA_HIP_EINVAL, "no such completed request");
-		arma_hip::free_request(s, it->second);
+		arma_hip::Request* r = it->second;
 		s->finished.erase(it);
-		s->cv_sched.notify_all();
+		// the lane that made it frees its memory (on its own thread and stream)
+		arma_hip::Lane* L = s->lanes[std::max(r->lane, 0)].get();
+		L->frees.push_back(r);
+		L->cv.notify_one();
 	});
 }
 
@@ -597,8 +631,11 @@ void arma_hip_server_destroy(arma_hip_server* s) {
 		if (r->stream) --r->stream->pending;
 		delete r;
 	}
-	for (arma_hip::Request* r : s->done) arma_hip::free_request(s, r);
-	for (auto& kv : s->finished) arma_hip::free_request(s, kv.second);
+	auto lane_stream = [&](arma_hip::Request* r) { return s->lanes[std::max(r->lane, 0)]->ctx->stream; };
+	for (arma_hip::Request* r : s->done) arma_hip::free_request(lane_stream(r), r);
+	for (auto& kv : s->finished) arma_hip::free_request(lane_stream(kv.second), kv.second);
+	for (auto& L : s->lanes)
+		for (arma_hip::Request* r : L->frees) arma_hip::free_request(L->ctx->stream, r);
 	for (auto& L : s->lanes) (void)hipStreamSynchronize(L->ctx->stream);
 	for (auto& kv : s->streams) (void)hipFree(kv.second->history);
 	for (auto& L : s->lanes) {

@@ -56,6 +56,15 @@ WORKLOADS = {
     "batch-64-fp32": dict(kind="batch", count=64, seed=42, t=(64, 1024), xy=(32, 384),
                           orders=(3, 5, 7, 9), alpha=(0.04, 0.1), beta=(0.5, 1.2),
                           partition=(64, 64, 64), precision="fp32"),
+    # serving: Poisson arrivals at 100 requests/s for 20 s (seed 7); surface
+    # volumes log-uniform in [32^3, 512x384x384]; a quarter of the requests
+    # are 16-slice chunks of 8 realtime streams; a fifth of the surfaces also
+    # ask for the linear potential at 2 z levels; FP32; 200 GB HBM budget
+    "serve-100qps-fp32": dict(kind="serve", qps=100.0, duration=20.0, seed=7,
+                              volume=(32 ** 3, 512 * 384 * 384), t=(32, 512), xy=(32, 384),
+                              stream_share=0.25, streams=8, chunk=16, potential_share=0.2,
+                              nz=2, budget=200 << 30, lanes=4, max_batch=32,
+                              partition=(64, 64, 64), precision="fp32"),
     # CPU-runnable plain AR case (no potential)
     "ar-standing-128x64x64-fp64": dict(
         acf=("standing", 0.8, (0.8, 0.8)), shape=(128, 64, 64), order=7,
@@ -200,6 +209,147 @@ def run_batch(args, w):
     ctx.close()
 
 
+def serve_requests(w):
+    """The request trace of a serving workload: arrival times and shapes."""
+    import numpy as np
+    rng = np.random.default_rng(w["seed"])
+    t, out = 0.0, []
+    lo, hi = math.log(w["volume"][0]), math.log(w["volume"][1])
+    while True:
+        t += float(rng.exponential(1.0 / w["qps"]))
+        if t > w["duration"]:
+            break
+        if rng.random() < w["stream_share"]:
+            out.append(dict(t=t, stream=int(rng.integers(w["streams"]))))
+            continue
+        v = math.exp(rng.uniform(lo, hi))
+        pot = bool(rng.random() < w["potential_share"])
+        # aspect uniform on the feasible set: log nt, then log nx, the rest is ny
+        tl, th = w["t"]
+        xl, xh = w["xy"]
+        nt_lo, nt_hi = max(tl, v / (xh * xh)), min(th, v / (xl * xl))
+        nt = int(round(math.exp(rng.uniform(math.log(nt_lo), math.log(nt_hi)))))
+        rest = v / nt
+        nx_lo, nx_hi = max(xl, rest / xh), min(xh, rest / xl)
+        nx = int(round(math.exp(rng.uniform(math.log(nx_lo), math.log(max(nx_lo, nx_hi))))))
+        ny = int(min(xh, max(xl, round(rest / nx))))
+        if pot:  # the FFT potential takes power-of-two (x, y) extents
+            nx = 1 << int(round(math.log2(min(256, max(32, nx)))))
+            ny = 1 << int(round(math.log2(min(256, max(32, ny)))))
+        out.append(dict(t=t, shape=(nt, nx, ny), potential=pot))
+    return out
+
+
+def run_serve(args, w):
+    """Serving workload: a client thread submits the trace at its arrival
+    times; the main thread polls completions and releases them at once.
+    value = (surface + potential points) per second from the first arrival
+    to the last completion; latency = completion - arrival."""
+    import threading
+    import numpy as np
+    import arma_amd
+    from arma_amd import _abi
+    trace = serve_requests(w)
+    wl = dict(acf=("propagating", 0.8, (-0.8, 0.0)), order=7)
+    acf, phi, varwn = setup_model(arma_amd, wl)
+    gamma0 = float(acf[0, 0, 0])
+    part = w["partition"]
+    prec = arma_amd.FP32
+    ctx_desc = arma_amd.Context  # descriptors only: built with the static helper
+    params = _abi.dcmt_p521_params(4096)
+    rng = np.random.default_rng(w["seed"] + 1)
+    log(f"bench: {len(trace)} requests over {w['duration']} s")
+    srv = arma_amd.Server(0, lanes=w["lanes"], budget=w["budget"], precision=prec,
+                          max_batch=w["max_batch"])
+    sids = []
+    for i in range(w["streams"]):
+        nx, ny = (int(v) for v in rng.integers(w["xy"][0], w["xy"][1] + 1, 2))
+        ncol = math.ceil(nx / part[1]) * math.ceil(ny / part[2])
+        d = ctx_desc.ar_desc((1, nx, ny), phi, varwn, [params[j] for j in range(ncol)],
+                             seeds=[10000 + 1000 * i], noise_mode=arma_amd.NOISE_PARTITIONED,
+                             partition=(w["chunk"], part[1], part[2]), precision=prec,
+                             algorithm=arma_amd.AR_WAVEFRONT)
+        sids.append((srv.open_stream(d, w["chunk"]), nx * ny))
+    descs = []
+    for i, r in enumerate(trace):
+        if "shape" not in r:
+            descs.append(None)
+            continue
+        n = math.prod(math.ceil(r["shape"][k] / part[k]) for k in range(3))
+        d = ctx_desc.ar_desc(r["shape"], phi, varwn, [params[j % 4096] for j in range(n)],
+                             seeds=np.full(n, i, np.uint32),
+                             noise_mode=arma_amd.NOISE_PARTITIONED, partition=part,
+                             precision=prec, algorithm=arma_amd.AR_WAVEFRONT)
+        pd = None
+        if r["potential"]:
+            pd = ctx_desc.pot_desc(r["shape"], (2 * math.pi,) * 3, 12.0, (0.0, -10.0),
+                                   (float(r["shape"][0] - 1), 0.0), (r["shape"][0], w["nz"]),
+                                   prec)
+        descs.append((d, pd))
+    pts = {}
+    errors = []
+
+    def client(t0):
+        for i, r in enumerate(trace):
+            delay = t0 + r["t"] - arma_amd.Server.clock()
+            if delay > 0:
+                time.sleep(delay)
+            try:
+                if descs[i] is None:
+                    sid, area = sids[r["stream"]]
+                    rid = srv.submit_chunk(sid, arrival=t0 + r["t"])
+                    pts[rid] = w["chunk"] * area
+                else:
+                    d, pd = descs[i]
+                    rid = srv.submit_surface(d, gamma0, pd, arrival=t0 + r["t"])
+                    n = math.prod(r["shape"])
+                    pts[rid] = n + (n * w["nz"] if pd is not None else 0)
+            except Exception as e:
+                errors.append(str(e))
+
+    t0 = arma_amd.Server.clock() + 0.5
+    th = threading.Thread(target=client, args=(t0,))
+    th.start()
+    lat, sizes, done_pts, t_last, last_log = [], [], 0, t0, t0
+    while len(lat) + len(errors) < len(trace):
+        for c in srv.poll(256, wait_ms=1000):
+            if c.status != 0:
+                errors.append(f"request {c.id} status {c.status}")
+            else:
+                lat.append(c.t_done - c.t_arrival)
+                sizes.append(c.batch_size)
+                t_last = max(t_last, c.t_done)
+                done_pts += pts.get(c.id, 0)
+            srv.release(c.id)
+        now = arma_amd.Server.clock()
+        if now - last_log > 5:
+            log(f"bench: {len(lat)} of {len(trace)} requests served")
+            last_log = now
+    th.join()
+    st = srv.stats()
+    for sid, _ in sids:
+        srv.close_stream(sid)
+    srv.close()
+    lat = np.array(lat) * 1000.0
+    wall = t_last - t0
+    line = {
+        "metric": "surface pts/s + potential pts/s (FP32 vs FP64) at RMS<=1e-5; p50/p99 latency",
+        "value": done_pts / wall, "unit": "pts/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": wall * 1000.0, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": args.workload, "qps_offered": w["qps"],
+                   "duration_s": w["duration"], "requests": len(trace),
+                   "lanes": w["lanes"], "max_batch": w["max_batch"],
+                   "hbm_budget_bytes": w["budget"], "parallelism": "single-gpu"},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                       "mean": float(lat.mean()), "max": float(lat.max())},
+        "qps_served": len(lat) / wall, "errors": errors[:5], "n_errors": len(errors),
+        "batch_size": {"mean": float(np.mean(sizes)), "max": int(np.max(sizes))},
+        "server": st, "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def sample_shape(w):
     """The bounded sample the CPU oracle runs: the first slices of the surface
     (two partition layers, or the whole surface when small) and the potential
@@ -294,6 +444,9 @@ def main():
     ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
     ap.add_argument("--ar", default="wavefront", choices=["wavefront", "plain"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="after the run, one device-to-device copy of this many MiB "
+                         "(a known-size kernel for scaling hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the AR path shards by "
@@ -309,6 +462,9 @@ def main():
     w = WORKLOADS[args.workload]
     if w.get("kind") == "batch":
         run_batch(args, w)
+        return
+    if w.get("kind") == "serve":
+        run_serve(args, w)
         return
     fp32 = w["precision"] == "fp32"
     prec = arma_amd.FP32 if fp32 else arma_amd.FP64
@@ -449,6 +605,14 @@ def main():
     }
     print(json.dumps(line), flush=True)
     ctx.close()
+    if args.copy_calibration_mb:
+        n = args.copy_calibration_mb << 20
+        src = torch.ones(n // 4, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"bench: copied {args.copy_calibration_mb} MiB device to device")
 
 
 if __name__ == "__main__":
