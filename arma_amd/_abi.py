@@ -43,6 +43,7 @@ EXPORTS = [
     "arma_hip_server_create", "arma_hip_server_destroy", "arma_hip_server_open_stream",
     "arma_hip_server_close_stream", "arma_hip_server_submit", "arma_hip_server_poll",
     "arma_hip_server_release", "arma_hip_server_stats", "arma_hip_server_copy",
+    "arma_hip_yule_walker_choi_device", "arma_hip_white_noise_variance_device",
 ]
 
 
@@ -207,6 +208,10 @@ def lib():
         L.arma_hip_server_release.argtypes = [vp, C.c_int64]
         L.arma_hip_server_stats.argtypes = [vp, C.POINTER(ServerStats)]
         L.arma_hip_server_copy.argtypes = [vp, vp, vp, C.c_size_t]
+        L.arma_hip_yule_walker_choi_device.argtypes = [vp, dp, C.c_int32, C.c_int32, C.c_double,
+                                                       C.c_int32, C.c_int32, C.c_double, dp,
+                                                       i3, dp]
+        L.arma_hip_white_noise_variance_device.argtypes = [vp, dp, i3, dp, i3, dp]
         L.arma_hip_profile.argtypes = [vp, C.c_int32]
         L.arma_hip_profile_report.argtypes = [vp, C.c_char_p, C.c_size_t, C.c_int32]
         _lib = L
@@ -432,6 +437,30 @@ class Context:
         check(lib().arma_hip_mt_raw(self.h, C.cast(marr, C.POINTER(MtParams)),
                                     None if sarr is None else sarr.ctypes.data, len(marr), draws,
                                     C.c_void_p(_ptr(out))))
+
+    def yule_walker_choi(self, acf_cube, max_order, var_eps=1e-6, determine_order=True,
+                         chop=True, chop_eps=1e-10):
+        """Choi coefficients in FP64 on this context's GPU: (phi, solver variance)."""
+        acf_cube = np.ascontiguousarray(acf_cube, np.float64)
+        n = acf_cube.shape[0]
+        out = np.zeros(max(max_order, 2) ** 3 + n ** 3, np.float64)
+        order = (C.c_int32 * 3)()
+        var = C.c_double()
+        check(lib().arma_hip_yule_walker_choi_device(
+            self.h, acf_cube.reshape(-1), n, max_order, var_eps, 1 if determine_order else 0,
+            1 if chop else 0, chop_eps, out, order, C.byref(var)))
+        shape = tuple(order)
+        return out[:int(np.prod(shape))].reshape(shape).copy(), var.value
+
+    def white_noise_variance(self, acf_cube, phi):
+        """AR white-noise variance in FP64 on this context's GPU."""
+        acf_cube = np.ascontiguousarray(acf_cube, np.float64)
+        phi = np.ascontiguousarray(phi, np.float64)
+        out = C.c_double()
+        check(lib().arma_hip_white_noise_variance_device(
+            self.h, acf_cube.reshape(-1), (C.c_int32 * 3)(*acf_cube.shape), phi.reshape(-1),
+            (C.c_int32 * 3)(*phi.shape), C.byref(out)))
+        return out.value
 
     def batch(self, descs, gamma0=None, precision=FP32, config=(0, 0, 0), history=None):
         return Batch(self, descs, gamma0, precision, config, history)

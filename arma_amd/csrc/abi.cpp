@@ -161,7 +161,7 @@ void arma_hip_destroy(arma_hip_ctx* ctx) {
 	if (!ctx) return;
 	(void)hipSetDevice(ctx->device);
 	(void)hipStreamSynchronize(ctx->stream);
-	for (arma_hip::Scratch* s : {&ctx->params, &ctx->seeds, &ctx->state, &ctx->phi,
+	for (arma_hip::Scratch* s : {&ctx->yw, &ctx->params, &ctx->seeds, &ctx->state, &ctx->phi,
 	                             &ctx->partials, &ctx->flags, &ctx->noise_tmp, &ctx->mbox, &ctx->pot_spec,
 	                             &ctx->pot_mid, &ctx->pot_wn, &ctx->pot_twiddle, &ctx->pot_kmin,
 	                             &ctx->zeta_tmp, &ctx->phi_tmp})
@@ -632,6 +632,44 @@ int arma_hip_batch_factors(arma_hip_ctx* ctx, arma_hip_batch* batch, double* out
 		use_device(ctx);
 		ARMA_HIP_REQUIRE(batch && out, ARMA_HIP_EINVAL, "null argument");
 		arma_hip::rescale_batch_factors(ctx, batch->rescale, out);
+	});
+}
+
+int arma_hip_yule_walker_choi_device(arma_hip_ctx* ctx, const double* acf, int32_t n,
+                                     int32_t max_order, double var_eps, int32_t determine_order,
+                                     int32_t do_chop, double chop_eps, double* phi_out,
+                                     int32_t order_out[3], double* varwn_solver) {
+	return guarded([&] {
+		use_device(ctx);
+		ARMA_HIP_REQUIRE(acf && phi_out && order_out && n >= 3, ARMA_HIP_EINVAL,
+		                 "bad Yule-Walker argument");
+		std::vector<double> phi;
+		int l = 0;
+		double var = 0.0;
+		arma_hip::yule_walker_choi_device(ctx, acf, n, max_order, var_eps, determine_order != 0,
+		                                  phi, l, var);
+		int shp[3] = {l, l, l};
+		if (do_chop) arma_hip::chop_shape(phi, shp, chop_eps, shp);
+		for (int a = 0; a < shp[0]; ++a)
+			for (int b = 0; b < shp[1]; ++b)
+				for (int c = 0; c < shp[2]; ++c)
+					phi_out[(size_t(a) * shp[1] + b) * shp[2] + c] = phi[(size_t(a) * l + b) * l + c];
+		for (int k = 0; k < 3; ++k) order_out[k] = shp[k];
+		if (varwn_solver) *varwn_solver = var;
+	});
+}
+
+int arma_hip_white_noise_variance_device(arma_hip_ctx* ctx, const double* acf,
+                                         const int32_t acf_shape[3], const double* phi,
+                                         const int32_t phi_shape[3], double* out) {
+	return guarded([&] {
+		use_device(ctx);
+		ARMA_HIP_REQUIRE(acf && acf_shape && phi && phi_shape && out, ARMA_HIP_EINVAL,
+		                 "null argument");
+		for (int k = 0; k < 3; ++k)
+			ARMA_HIP_REQUIRE(phi_shape[k] > 0 && phi_shape[k] <= acf_shape[k], ARMA_HIP_EINVAL,
+			                 "AR order larger than the ACF grid");
+		*out = arma_hip::white_noise_variance_device(ctx, acf, acf_shape, phi, phi_shape);
 	});
 }
 

@@ -89,6 +89,7 @@ struct arma_hip_ctx {
 	// named scratch areas
 	arma_hip::Scratch params, seeds, state, phi, partials, flags, noise_tmp, mbox;
 	arma_hip::Scratch pot_spec, pot_mid, pot_wn, pot_twiddle, pot_kmin, zeta_tmp, phi_tmp;
+	arma_hip::Scratch yw;
 	int twiddle_n = 0;
 	// batch configurations chosen by the autotuner, by batch signature:
 	// tile width, schedule, workgroups per CU
@@ -139,6 +140,16 @@ void launch_ar_plain(arma_hip_ctx* ctx, const arma_ar_desc* d, void* zeta);
 void launch_ar_wavefront(arma_hip_ctx* ctx, const arma_ar_desc* d, void* zeta);
 double launch_rescale(arma_hip_ctx* ctx, void* zeta, const int32_t shape[3], int precision,
                       double gamma0);
+
+// FP64 Yule-Walker (Choi) and white-noise variance on the GPU.  phi gets
+// the l x l x l coefficients before chopping.
+void yule_walker_choi_device(arma_hip_ctx* ctx, const double* acf, int n, int max_order,
+                             double var_eps, bool determine, std::vector<double>& phi, int& l,
+                             double& varwn);
+double white_noise_variance_device(arma_hip_ctx* ctx, const double* acf, const int32_t acf_shape[3],
+                                   const double* phi, const int32_t phi_shape[3]);
+// chop (src/chop.cc:4-26) of an n0 x n1 x n2 array: the trimmed shape.
+void chop_shape(const std::vector<double>& phi, const int shape[3], double eps, int out[3]);
 
 // ---- batches of surfaces in one arena ----
 

@@ -265,6 +265,11 @@ void chop(const std::vector<double>& phi, const int shape[3], double eps, int ou
 
 }  // namespace
 
+void arma_hip::chop_shape(const std::vector<double>& phi, const int shape[3], double eps,
+                          int out[3]) {
+	chop(phi, shape, eps, out);
+}
+
 extern "C" {
 
 int arma_hip_acf(int32_t func, const double delta[3], const int32_t shape[3], double amplitude,

@@ -55,7 +55,7 @@ WORKLOADS = {
     # autotuned tile shape / schedule / residency, launches replayed as a graph
     "batch-64-fp32": dict(kind="batch", count=64, seed=42, t=(64, 1024), xy=(32, 384),
                           orders=(3, 5, 7, 9), alpha=(0.04, 0.1), beta=(0.5, 1.2),
-                          partition=(64, 64, 64), precision="fp32"),
+                          partition=(32, 48, 48), precision="fp32"),
     # serving: Poisson arrivals at 100 requests/s for 20 s (seed 7); surface
     # volumes log-uniform in [32^3, 512x384x384]; a quarter of the requests
     # are 16-slice chunks of 8 realtime streams; a fifth of the surfaces also
@@ -64,7 +64,7 @@ WORKLOADS = {
                               volume=(32 ** 3, 512 * 384 * 384), t=(32, 512), xy=(32, 384),
                               stream_share=0.25, streams=8, chunk=16, potential_share=0.2,
                               nz=2, budget=200 << 30, lanes=4, max_batch=32,
-                              partition=(64, 64, 64), precision="fp32"),
+                              partition=(32, 48, 48), precision="fp32"),
     # CPU-runnable plain AR case (no potential)
     "ar-standing-128x64x64-fp64": dict(
         acf=("standing", 0.8, (0.8, 0.8)), shape=(128, 64, 64), order=7,
@@ -424,6 +424,19 @@ def cpu_baseline(w, phi, varwn, params, seeds, gpu_sample):
             "accuracy": acc}
 
 
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of a kernel from the committed counter summary
+    (profiles/hbm_traffic_*.txt: FETCH_SIZE and WRITE_SIZE runs scaled by a
+    known-size copy), or None when that workload was not measured."""
+    import glob
+    for path in glob.glob(os.path.join(ROOT, "profiles", "hbm_traffic_*.txt")):
+        for line in open(path):
+            f = line.split()
+            if len(f) == 4 and f[0] == "traffic" and f[1] == workload and f[2] == kernel:
+                return int(f[3])
+    return None
+
+
 def parse_profile(ctx):
     from arma_amd import _abi
     buf = C.create_string_buffer(1 << 16)
@@ -567,7 +580,7 @@ def main():
         k = kernels[dom]
         roofline = {"bound": k["bound"], "achieved": round(k["achieved"], 3), "peak": k["peak"],
                     "unit": k["unit"], "frac": round(k["achieved"] / k["peak"], 4),
-                    "traffic": None, "kernel": dom}
+                    "traffic": measured_traffic(args.workload, dom), "kernel": dom}
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -202,6 +202,18 @@ int arma_hip_yule_walker_gauss(const double* acf, const int32_t acf_shape[3],
 double arma_hip_white_noise_variance(const double* acf, const int32_t acf_shape[3],
                                      const double* phi, const int32_t phi_shape[3]);
 
+/* The same two setup steps in FP64 on the GPU of ctx (host inputs and
+   outputs).  The Choi recursion runs as one workgroup in the host solver's
+   order of operations; its coefficients, order and variance equal
+   arma_hip_yule_walker_choi's to the last bit. */
+int arma_hip_yule_walker_choi_device(arma_hip_ctx* ctx, const double* acf, int32_t n,
+                                     int32_t max_order, double var_eps, int32_t determine_order,
+                                     int32_t chop, double chop_eps, double* phi_out,
+                                     int32_t order_out[3], double* varwn_solver);
+int arma_hip_white_noise_variance_device(arma_hip_ctx* ctx, const double* acf,
+                                         const int32_t acf_shape[3], const double* phi,
+                                         const int32_t phi_shape[3], double* out);
+
 /* get_partition_shape (src/generator/ar_model.cc:23-50). */
 void arma_hip_partition_shape(const int32_t partition[3], const int32_t grid[3],
                               const int32_t order[3], int32_t parallelism, int32_t out[3]);

@@ -275,3 +275,31 @@ def test_partitioned_noise_large_state(hip_ctx):
     ref = O.noise_partitioned([to_oracle(p) for p in plist], seeds, shape, part, np.sqrt(0.5))
     u = float(ulp_diff(eps.cpu().numpy(), ref))
     assert u <= 4, u
+
+
+@pytest.mark.parametrize("kind,vel,beta,alpha,order,determine", [
+    (O.STANDING, 0.8, (0.8, 0.8), 0.06, 7, True),
+    (O.PROPAGATING, 0.8, (-0.8, 0.0), 0.06, 7, True),
+    (O.PROPAGATING, 1.1, (-1.1, 0.0), 0.06, 9, True),
+    (O.STANDING, 0.6, (0.9, 0.7), 0.08, 5, False),
+    (O.STANDING, 1.0, (0.5, 1.2), 0.05, 9, False),
+])
+def test_yule_walker_on_gpu(hip_ctx, kind, vel, beta, alpha, order, determine):
+    """FP64 Choi solve and white-noise variance on the GPU equal the host
+    solver's to the last bit (the host solver is pinned to the oracle and the
+    reference's Yule-Walker tests in test_oracle_yw_ar.py)."""
+    acf = O.acf(kind, (10, 10, 10), velocity=vel, beta=beta, alpha=(alpha,) * 3)
+    ph, var = _abi.yule_walker_choi(acf, order, determine_order=determine)
+    pg, varg = hip_ctx.yule_walker_choi(acf, order, determine_order=determine)
+    assert pg.shape == ph.shape
+    np.testing.assert_array_equal(pg, ph)
+    assert varg == var
+    assert hip_ctx.white_noise_variance(acf, pg) == _abi.white_noise_variance(acf, ph)
+    ref, _ = O.yw_choi(acf, order, determine_order=determine)
+    np.testing.assert_allclose(pg, ref, rtol=0, atol=1e-9 * np.abs(ref).max())
+
+
+def test_yule_walker_on_gpu_rejects_low_order(hip_ctx):
+    acf = O.acf(O.STANDING, (10, 10, 10))
+    with pytest.raises(ValueError):
+        hip_ctx.yule_walker_choi(acf, 1)
