@@ -210,8 +210,9 @@ def lib():
         L.arma_hip_server_copy.argtypes = [vp, vp, vp, C.c_size_t]
         L.arma_hip_yule_walker_choi_device.argtypes = [vp, dp, C.c_int32, C.c_int32, C.c_double,
                                                        C.c_int32, C.c_int32, C.c_double, dp,
-                                                       i3, dp]
-        L.arma_hip_white_noise_variance_device.argtypes = [vp, dp, i3, dp, i3, dp]
+                                                       i3, C.POINTER(C.c_double)]
+        L.arma_hip_white_noise_variance_device.argtypes = [vp, dp, i3, dp, i3,
+                                                           C.POINTER(C.c_double)]
         L.arma_hip_profile.argtypes = [vp, C.c_int32]
         L.arma_hip_profile_report.argtypes = [vp, C.c_char_p, C.c_size_t, C.c_int32]
         _lib = L

@@ -214,8 +214,9 @@ void AR_model<T>::read(std::istream& in) {
 }
 
 template <class T>
-void AR_model<T>::determine_coefficients() {
-	// src/generator/ar_model.cc:81-154
+void AR_model<T>::determine_coefficients(bool on_device) {
+	// src/generator/ar_model.cc:81-154; FP64 on the GPU when a generation
+	// follows (bit-identical to the host solver, which the dry run uses)
 	if (algorithm_ == AR_algorithm::Choi) {
 		const int n = acf_shape_[0];
 		if (acf_shape_[1] != n || acf_shape_[2] != n) throw std::invalid_argument("ACF is not square");
@@ -226,8 +227,13 @@ void AR_model<T>::determine_coefficients() {
 		                size_t(n) * n * n,
 		            0.0);
 		double var = 0.0;
-		check(arma_hip_yule_walker_choi(acf_.data(), n, max_order, 1e-6, 1, 1, 1e-10, phi_.data(),
-		                                phi_shape_.data(), &var));
+		if (on_device)
+			check(arma_hip_yule_walker_choi_device(ctx_->get(), acf_.data(), n, max_order, 1e-6, 1,
+			                                       1, 1e-10, phi_.data(), phi_shape_.data(),
+			                                       &var));
+		else
+			check(arma_hip_yule_walker_choi(acf_.data(), n, max_order, 1e-6, 1, 1, 1e-10,
+			                                phi_.data(), phi_shape_.data(), &var));
 		phi_.resize(size_t(phi_shape_[0]) * phi_shape_[1] * phi_shape_[2]);
 		order_ = phi_shape_;
 		std::clog << "New AR model order = (" << order_[0] << ',' << order_[1] << ','
@@ -237,8 +243,12 @@ void AR_model<T>::determine_coefficients() {
 		phi_.assign(size_t(order_[0]) * order_[1] * order_[2], 0.0);
 		check(arma_hip_yule_walker_gauss(acf_.data(), acf_shape_.data(), order_.data(), phi_.data()));
 	}
-	varwn_ = arma_hip_white_noise_variance(acf_.data(), acf_shape_.data(), phi_.data(),
-	                                       phi_shape_.data());
+	if (on_device)
+		check(arma_hip_white_noise_variance_device(ctx_->get(), acf_.data(), acf_shape_.data(),
+		                                           phi_.data(), phi_shape_.data(), &varwn_));
+	else
+		varwn_ = arma_hip_white_noise_variance(acf_.data(), acf_shape_.data(), phi_.data(),
+		                                       phi_shape_.data());
 }
 
 namespace {
@@ -318,11 +328,16 @@ Array3D<T> AR_model<T>::do_generate() {
 
 template <class T>
 void AR_model<T>::prepare() {
+	prepare(false);
+}
+
+template <class T>
+void AR_model<T>::prepare(bool on_device) {
 	// src/generator/basic_arma_model.cc:51-97 with the closed-form ACF
 	if (!(order_[0] > 0 && order_[1] > 0 && order_[2] > 0)) order_ = acf_shape_;
 	for (int d = 0; d < 3; ++d)
 		this->outgrid_.length[d] = acf_grid_.delta(d) * this->outgrid_.num_patches(d) * 2.0 * M_PI;
-	determine_coefficients();
+	determine_coefficients(on_device);
 	std::clog << "White noise variance = " << varwn_ << '\n';
 	if (validate_)
 		std::clog << "validate: the process root check needs GSL and is not part of the HIP build\n";
@@ -343,7 +358,7 @@ void AR_model<T>::describe(std::ostream& out) const {
 
 template <class T>
 Array3D<T> AR_model<T>::generate() {
-	prepare();
+	prepare(true);
 	return do_generate();
 }
 

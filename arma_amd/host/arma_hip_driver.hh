@@ -143,12 +143,13 @@ public:
 	void read(std::istream& in) override;
 	Array3D<T> generate() override;
 	void prepare() override;
+	void prepare(bool on_device);
 	void describe(std::ostream& out) const override;
 	const std::vector<double>& coefficients() const { return phi_; }
 	Shape3D order() const { return order_; }
 	double white_noise_variance() const { return varwn_; }
 	// device-resident surface of the last generate() (Array3D on the host)
-	void determine_coefficients();
+	void determine_coefficients(bool on_device);
 	Array3D<T> do_generate();
 
 private:

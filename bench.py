@@ -77,17 +77,19 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def setup_model(arma, w):
-    """FP64 host setup, as the reference generator does before do_generate."""
+def setup_model(arma, w, ctx):
+    """FP64 setup, as the reference generator does before do_generate: the
+    closed-form ACF on the host, the Choi solve and the white-noise variance
+    in FP64 on the GPU."""
     kind, velocity, beta = w["acf"]
     func = arma._abi.ACF_STANDING_WAVE if kind == "standing" else arma._abi.ACF_PROPAGATING_WAVE
     acf = arma._abi.acf(func, (10, 10, 10), velocity=velocity, alpha=(0.06, 0.06, 0.06), beta=beta)
-    phi, _ = arma._abi.yule_walker_choi(acf, w["order"])
-    varwn = arma._abi.white_noise_variance(acf, phi)
+    phi, _ = ctx.yule_walker_choi(acf, w["order"])
+    varwn = ctx.white_noise_variance(acf, phi)
     return acf, phi, varwn
 
 
-def batch_members(arma, w):
+def batch_members(arma, w, ctx):
     """The surfaces of a batch workload, drawn from a seeded generator.  A
     draw whose Choi coefficients give a negative white-noise variance (the
     reference rejects it with std::invalid_argument) is redrawn."""
@@ -104,8 +106,8 @@ def batch_members(arma, w):
             b = float(rng.uniform(*w["beta"]))
             acf = arma._abi.acf(arma._abi.ACF_PROPAGATING_WAVE, (10, 10, 10), velocity=b,
                                 alpha=(a, a, a), beta=(-b, 0.0))
-            phi, _ = arma._abi.yule_walker_choi(acf, order)
-            varwn = arma._abi.white_noise_variance(acf, phi)
+            phi, _ = ctx.yule_walker_choi(acf, order)
+            varwn = ctx.white_noise_variance(acf, phi)
             if varwn > 0:
                 break
         out.append(dict(shape=(nt, nx, ny), order=order, alpha=a, beta=b, phi=phi,
@@ -123,13 +125,13 @@ def run_batch(args, w):
     import arma_amd
     from arma_amd import _abi
     t0 = time.time()
-    mem = batch_members(arma_amd, w)
+    ctx = arma_amd.Context(0, torch_stream=False)
+    mem = batch_members(arma_amd, w, ctx)
     part = w["partition"]
     nstreams = [math.prod(math.ceil(m["shape"][d] / part[d]) for d in range(3)) for m in mem]
     params = _abi.dcmt_p521_params(max(nstreams))
     log(f"bench: {len(mem)} surfaces set up in {time.time() - t0:.1f}s, "
         f"{sum(nstreams)} noise streams")
-    ctx = arma_amd.Context(0, torch_stream=False)
     prec = arma_amd.FP32 if w["precision"] == "fp32" else arma_amd.FP64
     tdt = torch.float32 if prec == arma_amd.FP32 else torch.float64
     descs = [ctx.ar_desc(m["shape"], m["phi"], m["varwn"], params,
@@ -251,7 +253,9 @@ def run_serve(args, w):
     from arma_amd import _abi
     trace = serve_requests(w)
     wl = dict(acf=("propagating", 0.8, (-0.8, 0.0)), order=7)
-    acf, phi, varwn = setup_model(arma_amd, wl)
+    sctx = arma_amd.Context(0, torch_stream=False)
+    acf, phi, varwn = setup_model(arma_amd, wl, sctx)
+    sctx.close()
     gamma0 = float(acf[0, 0, 0])
     part = w["partition"]
     prec = arma_amd.FP32
@@ -485,7 +489,8 @@ def main():
     esize = 4 if fp32 else 8
     nt, nx, ny = w["shape"]
     log(f"bench: workload {args.workload}")
-    acf, phi, varwn = setup_model(arma_amd, w)
+    ctx = arma_amd.Context(0)
+    acf, phi, varwn = setup_model(arma_amd, w, ctx)
     log(f"bench: phi shape {phi.shape}, white-noise variance {varwn:.6g}")
     part = w["partition"]
     nstreams = math.prod(math.ceil(w["shape"][d] / part[d]) for d in range(3))
@@ -494,7 +499,6 @@ def main():
     log(f"bench: {nstreams} MT parameter sets ready in {time.time() - t0:.1f}s")
     seeds = np.full(nstreams, w["seed"], np.uint32)
 
-    ctx = arma_amd.Context(0)
     algo = arma_amd.AR_WAVEFRONT if args.ar == "wavefront" else arma_amd.AR_PLAIN
     desc = ctx.ar_desc(w["shape"], phi, varwn, params, seeds=seeds,
                        noise_mode=arma_amd.NOISE_PARTITIONED, partition=part, precision=prec,
@@ -611,6 +615,7 @@ def main():
                    "phi_shape": list(phi.shape), "partition": list(part),
                    "noise_streams": nstreams, "ar_kernel": args.ar,
                    "potential": list(w["npoints"]) + [nx, ny] if has_pot else None,
+                   "setup": "FP64 Yule-Walker (Choi) and white-noise variance on the GPU",
                    "parallelism": "single-gpu"},
         "surface_pts_per_s": surf_rate, "potential_pts_per_s": pot_rate,
         "stage_ms_per_step": {k: v / args.steps for k, v in stage_ms.items()},
