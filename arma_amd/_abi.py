@@ -44,6 +44,7 @@ EXPORTS = [
     "arma_hip_server_close_stream", "arma_hip_server_submit", "arma_hip_server_poll",
     "arma_hip_server_release", "arma_hip_server_stats", "arma_hip_server_copy",
     "arma_hip_yule_walker_choi_device", "arma_hip_white_noise_variance_device",
+    "arma_hip_validate_process",
 ]
 
 
@@ -180,6 +181,7 @@ def lib():
         L.arma_hip_white_noise_variance.argtypes = [dp, i3, dp, i3]
         L.arma_hip_white_noise_variance.restype = C.c_double
         L.arma_hip_partition_shape.argtypes = [i3, i3, i3, C.c_int32, i3]
+        L.arma_hip_validate_process.argtypes = [dp, i3, C.POINTER(C.c_int32), vp, C.c_int32]
         L.arma_hip_partition_shape.restype = None
         L.arma_hip_batch_create.argtypes = [vp, C.POINTER(BatchDesc), C.POINTER(vp)]
         L.arma_hip_batch_destroy.argtypes = [vp]
@@ -322,6 +324,27 @@ def white_noise_variance(acf_cube, phi):
     p = np.ascontiguousarray(phi, np.float64)
     return lib().arma_hip_white_noise_variance(a.reshape(-1), (C.c_int32 * 3)(*a.shape),
                                                p.reshape(-1), (C.c_int32 * 3)(*p.shape))
+
+
+def validate_process(phi):
+    """validate_process (src/arma.cc:15-70): (number of roots of
+    1 - phi_1 x - ... inside or on the unit circle, the roots).  Raises
+    ArmaHipError (runtime error) as the reference does when that number is
+    not zero or the roots cannot be found."""
+    phi = np.ascontiguousarray(phi, np.float64)
+    n = phi.size
+    out = np.zeros(2 * max(n, 1), np.float64)
+    nbad = C.c_int32()
+    rc = lib().arma_hip_validate_process(phi.reshape(-1), (C.c_int32 * 3)(*phi.shape),
+                                         C.byref(nbad), out.ctypes.data, out.size)
+    roots = out[:2 * max(n - 1, 0)].reshape(-1, 2)
+    roots = roots[:, 0] + 1j * roots[:, 1]
+    if rc != ARMA_HIP_OK:
+        err = ArmaHipError(rc, lib().arma_hip_last_error().decode(errors="replace"))
+        err.n_bad = nbad.value
+        err.roots = roots
+        raise err
+    return nbad.value, roots
 
 
 def partition_shape(partition, grid, order, parallelism):

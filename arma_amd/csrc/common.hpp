@@ -148,6 +148,11 @@ void yule_walker_choi_device(arma_hip_ctx* ctx, const double* acf, int n, int ma
                              double& varwn);
 double white_noise_variance_device(arma_hip_ctx* ctx, const double* acf, const int32_t acf_shape[3],
                                    const double* phi, const int32_t phi_shape[3]);
+// Roots of c[0] + c[1] x + ... + c[n-1] x^(n-1) (eigenvalues of the balanced
+// companion matrix by Francis double-shift QR); ARMA_HIP_ERUNTIME when the
+// leading coefficient is zero or the iteration does not converge.
+int polynomial_roots(const std::vector<double>& c, std::vector<double>& re,
+                     std::vector<double>& im);
 // chop (src/chop.cc:4-26) of an n0 x n1 x n2 array: the trimmed shape.
 void chop_shape(const std::vector<double>& phi, const int shape[3], double eps, int out[3]);
 

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -263,7 +264,190 @@ void chop(const std::vector<double>& phi, const int shape[3], double eps, int ou
 	out[2] = k + 1;
 }
 
+// ---- roots of 1 - phi_1 x - phi_2 x^2 - ... (validate_process) ----
+
+// Eigenvalues of the upper Hessenberg matrix h (n x n, row-major, destroyed)
+// by the Francis double-shift QR iteration with deflation on small
+// subdiagonal entries; wr/wi receive real and imaginary parts.  Returns
+// false when an eigenvalue does not converge in 60 iterations.
+bool hessenberg_eigenvalues(std::vector<double>& h, int n, std::vector<double>& wr,
+                            std::vector<double>& wi) {
+	auto H = [&](int i, int j) -> double& { return h[size_t(i) * n + j]; };
+	wr.assign(n, 0.0);
+	wi.assign(n, 0.0);
+	double norm = 0.0;
+	for (int i = 0; i < n; ++i)
+		for (int j = std::max(i - 1, 0); j < n; ++j) norm += std::fabs(H(i, j));
+	const double eps = std::numeric_limits<double>::epsilon();
+	int hi = n - 1;
+	int its = 0;
+	double exshift = 0.0;  // shifts accumulated by exceptional steps
+	while (hi >= 0) {
+		// smallest l with a negligible subdiagonal entry H(l, l-1)
+		int l = hi;
+		for (; l > 0; --l) {
+			double s = std::fabs(H(l - 1, l - 1)) + std::fabs(H(l, l));
+			if (s == 0.0) s = norm;
+			if (std::fabs(H(l, l - 1)) <= eps * s) {
+				H(l, l - 1) = 0.0;
+				break;
+			}
+		}
+		const double x = H(hi, hi);
+		if (l == hi) {  // one real eigenvalue
+			wr[hi] = x + exshift;
+			--hi;
+			its = 0;
+			continue;
+		}
+		const double y = H(hi - 1, hi - 1);
+		const double w = H(hi, hi - 1) * H(hi - 1, hi);
+		if (l == hi - 1) {  // a 2 x 2 block: two eigenvalues
+			const double p = 0.5 * (y - x);
+			const double q = p * p + w;
+			const double z = std::sqrt(std::fabs(q));
+			const double xs = x + exshift;
+			if (q >= 0.0) {
+				const double zz = p + (p >= 0.0 ? z : -z);
+				wr[hi - 1] = wr[hi] = xs + zz;
+				if (zz != 0.0) wr[hi] = xs - w / zz;
+			} else {
+				wr[hi - 1] = wr[hi] = xs + p;
+				wi[hi - 1] = z;
+				wi[hi] = -z;
+			}
+			hi -= 2;
+			its = 0;
+			continue;
+		}
+		if (its == 60) return false;
+		double sx = x, sy = y, sw = w;
+		if (its == 10 || its == 20) {
+			// exceptional shift
+			exshift += x;
+			for (int i = 0; i <= hi; ++i) H(i, i) -= x;
+			const double s = std::fabs(H(hi, hi - 1)) + std::fabs(H(hi - 1, hi - 2));
+			sx = sy = 0.75 * s;
+			sw = -0.4375 * s * s;
+		}
+		++its;
+		// double-shift step on rows/columns l..hi: first column of
+		// (H - s1)(H - s2) from the trailing 2 x 2 block
+		int m = hi - 2;
+		double p = 0, q = 0, r = 0;
+		for (; m >= l; --m) {
+			const double z = H(m, m);
+			const double rr = sx - z, ss = sy - z;
+			p = (rr * ss - sw) / H(m + 1, m) + H(m, m + 1);
+			q = H(m + 1, m + 1) - z - rr - ss;
+			r = H(m + 2, m + 1);
+			const double s = std::fabs(p) + std::fabs(q) + std::fabs(r);
+			p /= s;
+			q /= s;
+			r /= s;
+			if (m == l) break;
+			const double u = std::fabs(H(m, m - 1)) * (std::fabs(q) + std::fabs(r));
+			const double v = std::fabs(p) * (std::fabs(H(m - 1, m - 1)) + std::fabs(z) +
+			                                 std::fabs(H(m + 1, m + 1)));
+			if (u <= eps * v) break;
+		}
+		for (int i = m + 2; i <= hi; ++i) {
+			H(i, i - 2) = 0.0;
+			if (i != m + 2) H(i, i - 3) = 0.0;
+		}
+		// chase the bulge with 3 x 3 (last: 2 x 2) Householder reflectors
+		// P = I - v v^T * 2 / v^T v, v = (p + s, q, r), s = sign(p) |(p, q, r)|
+		for (int k = m; k <= hi - 1; ++k) {
+			const bool three = k != hi - 1;
+			double xn = 1.0;
+			if (k != m) {
+				p = H(k, k - 1);
+				q = H(k + 1, k - 1);
+				r = three ? H(k + 2, k - 1) : 0.0;
+				xn = std::fabs(p) + std::fabs(q) + std::fabs(r);
+				if (xn == 0.0) continue;
+				p /= xn;
+				q /= xn;
+				r /= xn;
+			}
+			double sn = std::sqrt(p * p + q * q + r * r);
+			if (p < 0.0) sn = -sn;
+			if (k == m) {
+				if (l != m) H(k, k - 1) = -H(k, k - 1);
+			} else {
+				H(k, k - 1) = -sn * xn;
+			}
+			const double ps = p + sn;
+			const double vx = ps / sn, vy = q / sn, vz = r / sn;
+			const double q2 = q / ps, r2 = r / ps;
+			for (int c = k; c <= hi; ++c) {  // rows k..k+2
+				double t = H(k, c) + q2 * H(k + 1, c);
+				if (three) {
+					t += r2 * H(k + 2, c);
+					H(k + 2, c) -= t * vz;
+				}
+				H(k + 1, c) -= t * vy;
+				H(k, c) -= t * vx;
+			}
+			const int imax = std::min(k + 3, hi);
+			for (int row = l; row <= imax; ++row) {  // columns k..k+2
+				double t = vx * H(row, k) + vy * H(row, k + 1);
+				if (three) {
+					t += vz * H(row, k + 2);
+					H(row, k + 2) -= t * r2;
+				}
+				H(row, k + 1) -= t * q2;
+				H(row, k) -= t;
+			}
+		}
+	}
+	return true;
+}
+
 }  // namespace
+
+int arma_hip::polynomial_roots(const std::vector<double>& c, std::vector<double>& re,
+                               std::vector<double>& im) {
+	// companion matrix of c[0] + c[1] x + ... + c[n-1] x^(n-1), balanced by
+	// powers of two (exact) row/column scalings before the QR iteration
+	const int n = int(c.size()) - 1;
+	if (n < 1) return ARMA_HIP_OK;
+	if (c[n] == 0.0) return ARMA_HIP_ERUNTIME;  // leading term must be non-zero
+	std::vector<double> h(size_t(n) * n, 0.0);
+	for (int j = 0; j < n; ++j) h[j] = -c[n - 1 - j] / c[n];
+	for (int i = 1; i < n; ++i) h[size_t(i) * n + i - 1] = 1.0;
+	for (bool done = false; !done;) {
+		done = true;
+		for (int i = 0; i < n; ++i) {
+			double rn = 0.0, cn = 0.0;
+			for (int j = 0; j < n; ++j)
+				if (j != i) {
+					rn += std::fabs(h[size_t(i) * n + j]);
+					cn += std::fabs(h[size_t(j) * n + i]);
+				}
+			if (rn == 0.0 || cn == 0.0) continue;
+			double f = 1.0;
+			const double sum = rn + cn;
+			while (cn < rn / 2.0) {
+				cn *= 2.0;
+				rn /= 2.0;
+				f *= 2.0;
+			}
+			while (cn >= rn * 2.0) {
+				cn /= 2.0;
+				rn *= 2.0;
+				f /= 2.0;
+			}
+			if ((rn + cn) < 0.95 * sum * 1.0 && f != 1.0) {
+				done = false;
+				for (int j = 0; j < n; ++j) h[size_t(i) * n + j] /= f;
+				for (int j = 0; j < n; ++j) h[size_t(j) * n + i] *= f;
+			}
+		}
+	}
+	if (!hessenberg_eigenvalues(h, n, re, im)) return ARMA_HIP_ERUNTIME;
+	return ARMA_HIP_OK;
+}
 
 void arma_hip::chop_shape(const std::vector<double>& phi, const int shape[3], double eps,
                           int out[3]) {
@@ -382,6 +566,45 @@ double arma_hip_white_noise_variance(const double* acf, const int32_t acf_shape[
 				s += phi[(size_t(a) * phi_shape[1] + b) * phi_shape[2] + c] *
 				     acf[(size_t(a) * acf_shape[1] + b) * acf_shape[2] + c];
 	return acf[0] - s;
+}
+
+int arma_hip_validate_process(const double* phi, const int32_t shape[3], int32_t* n_bad,
+                              double* roots, int32_t roots_len) {
+	try {
+		if (!phi || !shape || !n_bad) return ARMA_HIP_EINVAL;
+		*n_bad = 0;
+		const size_t n = size_t(shape[0]) * shape[1] * shape[2];
+		if (n <= 1) return ARMA_HIP_OK;
+		// 1 - phi_1 x - phi_2 x^2 - ... over the flattened coefficients
+		std::vector<double> c(n);
+		for (size_t i = 0; i < n; ++i) c[i] = -phi[i];
+		c[0] = 1.0;
+		std::vector<double> re, im;
+		if (arma_hip::polynomial_roots(c, re, im) != ARMA_HIP_OK) {
+			arma_hip::set_last_error(
+				"Can not find roots of the polynomial to verify AR/MA model "
+				"stationarity/invertibility.");
+			return ARMA_HIP_ERUNTIME;
+		}
+		int bad = 0;
+		for (size_t i = 0; i < re.size(); ++i) {
+			if (!(std::hypot(re[i], im[i]) > 1.0)) ++bad;
+			if (roots && int32_t(2 * i + 1) < roots_len) {
+				roots[2 * i] = re[i];
+				roots[2 * i + 1] = im[i];
+			}
+		}
+		*n_bad = bad;
+		if (bad > 0) {
+			arma_hip::set_last_error(
+				"AR/MA process is not stationary/invertible: some roots lie inside unit circle "
+				"or on its borderline.");
+			return ARMA_HIP_ERUNTIME;
+		}
+		return ARMA_HIP_OK;
+	} catch (...) {
+		return ARMA_HIP_ERUNTIME;
+	}
 }
 
 void arma_hip_partition_shape(const int32_t partition[3], const int32_t grid[3],

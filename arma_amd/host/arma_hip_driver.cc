@@ -339,8 +339,11 @@ void AR_model<T>::prepare(bool on_device) {
 		this->outgrid_.length[d] = acf_grid_.delta(d) * this->outgrid_.num_patches(d) * 2.0 * M_PI;
 	determine_coefficients(on_device);
 	std::clog << "White noise variance = " << varwn_ << '\n';
-	if (validate_)
-		std::clog << "validate: the process root check needs GSL and is not part of the HIP build\n";
+	// src/generator/basic_arma_model.cc:93-97 -> src/arma.cc:15-70
+	if (validate_) {
+		int32_t nbad = 0;
+		check(arma_hip_validate_process(phi_.data(), phi_shape_.data(), &nbad, nullptr, 0));
+	}
 }
 
 template <class T>

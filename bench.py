@@ -42,6 +42,12 @@ WORKLOADS = {
         acf=("propagating", 0.8, (-0.8, 0.0)), shape=(2048, 512, 512), order=7,
         partition=(32, 64, 64), seed=1, depth=12.0, lbound=(0.0, -10.0),
         ubound=(2047.0, 0.0), npoints=(2048, 32), precision="fp32"),
+    # the same surface and potential in FP64: with --ar plain this is the
+    # plain GPU path the FP32 path is compared with at equal size
+    "ar-potential-2048x512x512-fp64": dict(
+        acf=("propagating", 0.8, (-0.8, 0.0)), shape=(2048, 512, 512), order=7,
+        partition=(32, 64, 64), seed=1, depth=12.0, lbound=(0.0, -10.0),
+        ubound=(2047.0, 0.0), npoints=(2048, 32), precision="fp64"),
     # precision check on a stable model: the 2048x512x512 FP32 path with the
     # standing-wave ACF (the propagating ACF's Choi coefficients grow without
     # bound, so relative errors there measure the growth, not the precision)

@@ -214,6 +214,15 @@ int arma_hip_white_noise_variance_device(arma_hip_ctx* ctx, const double* acf,
                                          const int32_t acf_shape[3], const double* phi,
                                          const int32_t phi_shape[3], double* out);
 
+/* validate_process (src/arma.cc:15-70): the roots of 1 - phi_1 x - phi_2 x^2
+   - ... over the row-major coefficients (phi(0,0,0) replaced by the 1) must
+   all lie outside the unit circle.  n_bad receives the number that do not;
+   roots (optional) receives up to roots_len/2 (re, im) pairs.  Returns
+   ARMA_HIP_ERUNTIME with the reference's message when a root is inside or on
+   the circle, or when the roots cannot be found. */
+int arma_hip_validate_process(const double* phi, const int32_t shape[3], int32_t* n_bad,
+                              double* roots, int32_t roots_len);
+
 /* get_partition_shape (src/generator/ar_model.cc:23-50). */
 void arma_hip_partition_shape(const int32_t partition[3], const int32_t grid[3],
                               const int32_t order[3], int32_t parallelism, int32_t out[3]);
