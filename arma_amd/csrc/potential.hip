@@ -558,27 +558,31 @@ __device__ __forceinline__ void row_task(cpx<T>* X, int nx, int ny, int x0,
 	const int tid = threadIdx.x;
 	if (tid < R * N1) {
 		const int r = tid % R, n1 = tid / R;
-		const int x = x0 + r;
-		cpx<T> v[N2];
+		// rows past nx (none for the launch shapes used) transform a copy of
+		// the last row and are not stored
+		const int x = min(x0 + r, nx - 1);
+		// every load is issued before any arithmetic, with no branch between
+		// them, so that all of them are in flight at once
+		cpx<T> hk[N2], hm[N2];
+		double2 tk[N2];
 #pragma unroll
 		for (int n2 = 0; n2 < N2; ++n2) {
 			const int k = n1 + N1 * n2;
-			v[n2] = cpx<T>{T(0), T(0)};
-			if (x < nx) {
-				if (k == 0) {
-					// packed column: H[0] = real part, H[M] = imaginary part
-					const cpx<T> p = g[x];
-					v[n2] = cpx<T>{p.x + p.y, p.x - p.y};
-				} else {
-					// Z[k] = (H[k] + conj H[M-k]) + i e^{i pi k/M} (H[k] - conj H[M-k])
-					const cpx<T> hk = g[size_t(k) * nx + x];
-					const cpx<T> hm = conj(g[size_t(M - k) * nx + x]);
-					const cpx<T> sum = hk + hm;
-					const cpx<T> wd = twiddle<+1>(hk - hm, twy, k);
-					v[n2] = cpx<T>{sum.x - wd.y, sum.y + wd.x};
-				}
-			}
+			hk[n2] = g[size_t(k) * nx + x];
+			hm[n2] = g[size_t(k == 0 ? 0 : M - k) * nx + x];
+			tk[n2] = twy[k];
 		}
+		cpx<T> v[N2];
+#pragma unroll
+		for (int n2 = 0; n2 < N2; ++n2) {
+			// Z[k] = (H[k] + conj H[M-k]) + i e^{i pi k/M} (H[k] - conj H[M-k])
+			const cpx<T> a = hk[n2], b = conj(hm[n2]);
+			const cpx<T> sum = a + b;
+			const cpx<T> wd = cmul(a - b, T(tk[n2].x), T(-tk[n2].y));
+			v[n2] = cpx<T>{sum.x - wd.y, sum.y + wd.x};
+		}
+		// packed column k = 0: H[0] = real part, H[M] = imaginary part
+		if (n1 == 0) v[0] = cpx<T>{hk[0].x + hk[0].y, hk[0].x - hk[0].y};
 		dft<N2, +1>(v);
 		twiddle_out<N2, +1>(v, twy, 2 * n1,
 		                    [&](int k2, cpx<T> a) { X[xidx<N1, N2>(r, k2, n1)] = a; });
