@@ -32,7 +32,7 @@ EXPORTS = [
     "arma_hip_init", "arma_hip_destroy", "arma_hip_last_error", "arma_hip_set_stream",
     "arma_hip_get_stream", "arma_hip_synchronize", "arma_hip_white_noise",
     "arma_hip_ar_recurrence", "arma_hip_ar_generate", "arma_hip_rescale",
-    "arma_hip_linear_potential", "arma_hip_mt_raw", "arma_hip_read_mt_config",
+    "arma_hip_linear_potential", "arma_hip_high_amplitude_potential", "arma_hip_mt_raw", "arma_hip_read_mt_config",
     "arma_hip_write_mt_config", "arma_hip_dcmt_create", "arma_hip_dcmt_create_many",
     "arma_hip_acf", "arma_hip_yule_walker_choi", "arma_hip_yule_walker_gauss",
     "arma_hip_white_noise_variance", "arma_hip_partition_shape", "arma_hip_profile",
@@ -163,6 +163,7 @@ def lib():
         L.arma_hip_rescale.argtypes = [vp, vp, C.POINTER(C.c_int32), C.c_int32, C.c_double,
                                        C.POINTER(C.c_double)]
         L.arma_hip_linear_potential.argtypes = [vp, C.POINTER(PotDesc), vp, vp, C.c_int, vp]
+        L.arma_hip_high_amplitude_potential.argtypes = L.arma_hip_linear_potential.argtypes
         L.arma_hip_mt_raw.argtypes = [vp, C.POINTER(MtParams), vp, C.c_int32, C.c_int32, vp]
         L.arma_hip_read_mt_config.argtypes = [C.c_char_p, C.c_int32, C.POINTER(MtParams),
                                               C.POINTER(C.c_int32)]
@@ -446,14 +447,22 @@ class Context:
         d.precision = precision
         return d
 
-    def linear_potential(self, desc, zeta, phi, on_device=True, want_wnmax=False):
+    def linear_potential(self, desc, zeta, phi, on_device=True, want_wnmax=False,
+                         solver="linear"):
+        """Velocity potential of zeta into phi (nt_out, nz, nx, ny); solver is
+        "linear" or "high_amplitude" (the reference's registry keys)."""
+        fn = {"linear": lib().arma_hip_linear_potential,
+              "high_amplitude": lib().arma_hip_high_amplitude_potential}[solver]
         wn = None
         if want_wnmax:
             wn = np.empty(2 * desc.npoints[0], np.float64)
-        check(lib().arma_hip_linear_potential(self.h, C.byref(desc), C.c_void_p(_ptr(zeta)),
-                                              C.c_void_p(_ptr(phi)), 1 if on_device else 0,
-                                              None if wn is None else wn.ctypes.data))
+        check(fn(self.h, C.byref(desc), C.c_void_p(_ptr(zeta)), C.c_void_p(_ptr(phi)),
+                 1 if on_device else 0, None if wn is None else wn.ctypes.data))
         return None if wn is None else wn.reshape(-1, 2)
+
+    def high_amplitude_potential(self, desc, zeta, phi, on_device=True, want_wnmax=False):
+        return self.linear_potential(desc, zeta, phi, on_device, want_wnmax,
+                                     solver="high_amplitude")
 
     def mt_raw(self, mts, seeds, draws, out):
         marr = mts if isinstance(mts, C.Array) else params_array(list(mts))

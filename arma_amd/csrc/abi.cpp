@@ -163,7 +163,7 @@ void arma_hip_destroy(arma_hip_ctx* ctx) {
 	(void)hipStreamSynchronize(ctx->stream);
 	for (arma_hip::Scratch* s : {&ctx->yw, &ctx->params, &ctx->seeds, &ctx->state, &ctx->phi,
 	                             &ctx->partials, &ctx->flags, &ctx->noise_tmp, &ctx->mbox, &ctx->pot_spec,
-	                             &ctx->pot_mid, &ctx->pot_wn, &ctx->pot_twiddle, &ctx->pot_kmin,
+	                             &ctx->pot_mid, &ctx->pot_wn, &ctx->pot_twiddle, &ctx->pot_kmin, &ctx->pot_field,
 	                             &ctx->zeta_tmp, &ctx->phi_tmp})
 		s->release();
 	if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
@@ -239,8 +239,12 @@ int arma_hip_rescale(arma_hip_ctx* ctx, void* zeta, const int32_t shape[3], int3
 	});
 }
 
-int arma_hip_linear_potential(arma_hip_ctx* ctx, const arma_pot_desc* desc, const void* zeta,
-                              void* phi, int on_device, double* wnmax_out) {
+namespace {
+
+// Velocity_potential_solver<T>::operator() (src/velocity/basic_solver.cc:21-77)
+// for solver 0 (Linear_solver) or 1 (High_amplitude_solver).
+int potential_entry(arma_hip_ctx* ctx, const arma_pot_desc* desc, const void* zeta, void* phi,
+                    int on_device, double* wnmax_out, int solver) {
 	return guarded([&] {
 		use_device(ctx);
 		ARMA_HIP_REQUIRE(desc && zeta && phi, ARMA_HIP_EINVAL, "null argument");
@@ -265,12 +269,25 @@ int arma_hip_linear_potential(arma_hip_ctx* ctx, const arma_pot_desc* desc, cons
 			dz = tz;
 			dp = ctx->phi_tmp.get(pbytes);
 		}
-		arma_hip::launch_linear_potential(ctx, desc, dz, dp, wnmax_out);
+		arma_hip::launch_potential(ctx, desc, dz, dp, wnmax_out, solver);
 		if (!on_device) {
 			ARMA_HIP_CHECK(hipMemcpyAsync(phi, dp, pbytes, hipMemcpyDeviceToHost, ctx->stream));
 			ARMA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
 		}
 	});
+}
+
+}  // namespace
+
+int arma_hip_linear_potential(arma_hip_ctx* ctx, const arma_pot_desc* desc, const void* zeta,
+                              void* phi, int on_device, double* wnmax_out) {
+	return potential_entry(ctx, desc, zeta, phi, on_device, wnmax_out, 0);
+}
+
+int arma_hip_high_amplitude_potential(arma_hip_ctx* ctx, const arma_pot_desc* desc,
+                                      const void* zeta, void* phi, int on_device,
+                                      double* wnmax_out) {
+	return potential_entry(ctx, desc, zeta, phi, on_device, wnmax_out, 1);
 }
 
 int arma_hip_profile(arma_hip_ctx* ctx, int32_t enable) {

@@ -88,7 +88,7 @@ struct arma_hip_ctx {
 	int num_cus = 0;
 	// named scratch areas
 	arma_hip::Scratch params, seeds, state, phi, partials, flags, noise_tmp, mbox;
-	arma_hip::Scratch pot_spec, pot_mid, pot_wn, pot_twiddle, pot_kmin, zeta_tmp, phi_tmp;
+	arma_hip::Scratch pot_spec, pot_mid, pot_wn, pot_twiddle, pot_kmin, pot_field, zeta_tmp, phi_tmp;
 	arma_hip::Scratch yw;
 	int twiddle_n = 0;
 	// batch configurations chosen by the autotuner, by batch signature:
@@ -223,5 +223,8 @@ void rescale_batch_build(arma_hip_ctx* ctx, RescaleBatch* b, int precision,
 size_t ar_batch_member_bytes(const arma_ar_desc* d, int tile_x);
 void launch_linear_potential(arma_hip_ctx* ctx, const arma_pot_desc* d, const void* zeta,
                              void* phi, double* wnmax_out);
+// solver 0: Linear_solver, 1: High_amplitude_solver
+void launch_potential(arma_hip_ctx* ctx, const arma_pot_desc* d, const void* zeta, void* phi,
+                      double* wnmax_out, int solver);
 
 }  // namespace arma_hip

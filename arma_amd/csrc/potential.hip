@@ -1,4 +1,4 @@
-// Linear velocity potential on the GPU.
+// Linear and high-amplitude velocity potentials on the GPU.
 //
 // Reference: Velocity_potential_solver<T>::operator()
 // (src/velocity/basic_solver.cc:39-77) with Linear_solver<T>
@@ -32,6 +32,13 @@
 //    to the working precision.
 //  * Intermediate layouts are column-major ([column][x]) so that both the
 //    x-transforms and the row transforms read and write whole cache lines.
+//  * High_amplitude_solver (src/velocity/high_amplitude_solver.cc:6-23) feeds
+//    the same transforms the complex multiplier F = A + iB instead of the
+//    real -zeta_t.  A and B are real, so with W = Ws + Wa (even and odd parts
+//    of W_rot in k) Re(IFFT2((FFT2 A + i FFT2 B) W)) = C2R(FFT2(A) Ws +
+//    i FFT2(B) Wa): the product is Hermitian again, the half spectra of A
+//    and B are formed by the real-row path, and only the window step of
+//    the column pass differs.
 //  * factor_waves is a sequential run scan per slice; it runs as a chunked
 //    scan where each thread summarises a contiguous chunk of events and one
 //    thread folds the summaries in order (the fold is exact: representatives
@@ -86,6 +93,71 @@ __device__ __forceinline__ double neg_dzdt(const T* __restrict__ zeta, int nt, s
 	return -r;
 }
 
+// d zeta/dx (step = ny) or d zeta/dy (step = 1) at f = &zeta(it, x, y), c the
+// position along that dimension of extent n (src/derivative.hh:19-70).
+template <class T>
+__device__ __forceinline__ double dzeta_dxy(const T* __restrict__ f, int c, int n,
+                                            ptrdiff_t step, double d) {
+	if (n - 1 < 2) return double(f[0]);
+	if (c == 0)
+		return 0.5 * (-double(f[2 * step]) + 4.0 * double(f[step]) - 3.0 * double(f[0])) / d;
+	if (c == n - 1)
+		return 0.5 * (3.0 * double(f[0]) - 4.0 * double(f[-step]) + double(f[-2 * step])) / d;
+	return 0.5 * (double(f[step]) - double(f[-step])) / d;
+}
+
+// The multiplier of src/velocity/high_amplitude_solver.cc:14-22 at one point:
+// F = zt / (i ((zx + zy)/s - zx - zy) - 1/s), s = sqrt(1 + zx^2 + zy^2).
+// The complex division is Smith's algorithm (R. L. Smith, CACM 5(8), 1962),
+// the method of the compiler runtime's complex division on the host.
+__device__ __forceinline__ cpx<double> ha_multiplier(double zt, double zx, double zy) {
+	const double s = sqrt(1.0 + zx * zx + zy * zy);
+	const double c = 0.0 - 1.0 / s;             // real part of the denominator
+	const double d = (zx + zy) / s - zx - zy;   // imaginary part
+	if (fabs(c) >= fabs(d)) {
+		const double r = d / c, den = d * r + c;
+		return cpx<double>{zt / den, (0.0 - zt * r) / den};
+	}
+	const double r = c / d, den = c * r + d;
+	return cpx<double>{(zt * r) / den, (0.0 - zt) / den};
+}
+
+// The high-amplitude multiplier of every output slice as two real planes,
+// field[0][i][x][y] = Re F and field[1][i][x][y] = Im F, rounded to T (the
+// forward transform reads them in place of -zeta_t).  One thread per point,
+// consecutive threads along y.
+template <class T>
+__global__ __launch_bounds__(256) void ha_field_kernel(const T* __restrict__ zeta, int nt,
+                                                       int nx, int ny, int nto,
+                                                       const int* __restrict__ idx_t, double dt,
+                                                       double dx, double dy,
+                                                       T* __restrict__ field) {
+	const size_t slice = size_t(nx) * ny;
+	const size_t q = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	const int i = blockIdx.y;
+	if (q >= slice) return;
+	const int x = int(q / ny), y = int(q - size_t(x) * ny);
+	const int it = idx_t[i];
+	const T* f = zeta + size_t(it) * slice + q;
+	const cpx<double> m = ha_multiplier(-neg_dzdt(zeta, nt, slice, it, q, dt),
+	                                    dzeta_dxy(f, x, nx, ny, dx), dzeta_dxy(f, y, ny, 1, dy));
+	field[size_t(i) * slice + q] = T(m.x);
+	field[(size_t(nto) + i) * slice + q] = T(m.y);
+}
+
+// Value of the transformed real field at (it, x, y): -zeta_t of slice it
+// (the linear solver), or (src != nullptr) the plane value src[i][x][y] of
+// output slice i.
+template <class T>
+__device__ __forceinline__ double field_value(const T* __restrict__ zeta, int nt, int nx, int ny,
+                                              int it, int x, int y, double dt,
+                                              const T* __restrict__ src, int i) {
+	const size_t slice = size_t(nx) * ny;
+	const size_t off = size_t(x) * ny + y;
+	if (src) return double(src[size_t(i) * slice + off]);
+	return neg_dzdt(zeta, nt, slice, it, off, dt);
+}
+
 // Exchange buffer index of the four-step transform: sequence s, output k2 of
 // the first DFTs, input n1 of the second.  Rows are padded so that both the
 // writes (n1 or s varying across lanes) and the reads (k2 varying) spread
@@ -101,20 +173,21 @@ constexpr int xsize(int nseq) {
 
 // ---- forward spectrum -------------------------------------------------------
 
-// Rows of -zeta_t, real transform of length ny = 2M along y via a complex
-// transform of length M = N1*N2.  Writes the half spectrum k = 0..M
-// column-major: spec[i][k][x].  R rows per block, R*N1 <= 256.
+// Rows of the real field of field_value (-zeta_t, or the plane src), real
+// transform of length ny = 2M along y via a complex transform of length
+// M = N1*N2.  Writes the half spectrum k = 0..M column-major: spec[i][k][x].
+// R rows per block, R*N1 <= 256.
 template <class T, int N1, int N2>
 __global__ __launch_bounds__(kThreads) void fwd_rows_kernel(
 	const T* __restrict__ zeta, int nt, int nx, int ny, const int* __restrict__ idx_t, double dt,
-	const double2* __restrict__ twy, cpx<T>* __restrict__ spec, int R) {
+	const T* __restrict__ src, const double2* __restrict__ twy, cpx<T>* __restrict__ spec,
+	int R) {
 	constexpr int M = N1 * N2;
 	extern __shared__ unsigned char smem_raw[];
 	cpx<T>* X = reinterpret_cast<cpx<T>*>(smem_raw);
 	const int i = blockIdx.y;
 	const int x0 = blockIdx.x * R;
 	const int it = idx_t[i];
-	const size_t slice = size_t(nx) * ny;
 	const int tid = threadIdx.x;
 	if (tid < R * N1) {
 		const int n1 = tid % N1, r = tid / N1;
@@ -125,9 +198,8 @@ __global__ __launch_bounds__(kThreads) void fwd_rows_kernel(
 			const int n = n1 + N1 * n2;
 			v[n2] = cpx<T>{T(0), T(0)};
 			if (x < nx) {
-				const size_t off = size_t(x) * ny + 2 * n;
-				v[n2] = cpx<T>{T(neg_dzdt(zeta, nt, slice, it, off, dt)),
-				               T(neg_dzdt(zeta, nt, slice, it, off + 1, dt))};
+				v[n2] = cpx<T>{T(field_value(zeta, nt, nx, ny, it, x, 2 * n, dt, src, i)),
+				               T(field_value(zeta, nt, nx, ny, it, x, 2 * n + 1, dt, src, i))};
 			}
 		}
 		dft<N2, -1>(v);
@@ -459,22 +531,28 @@ template <class T, int N1, int N2>
 __device__ __forceinline__ void col_task(cpx<T>* X, int ny, int M, int nz, int i, int c0,
                                          const double2* __restrict__ twx,
                                          const cpx<T>* __restrict__ spec,
+                                         const cpx<T>* __restrict__ specb,
                                          const double* __restrict__ kmin, double z0, double dz,
                                          double h, cpx<T>* Gs, int CB, int ZB) {
 	constexpr int N = N1 * N2;
 	constexpr int EPT = N > kThreads ? N / kThreads : 1;  // CB*N <= max(256, N)
 	const int tid = threadIdx.x;
 	const int nel = CB * N;
-	const cpx<T>* base = spec + size_t(i) * (M + 1) * N;
-	cpx<T> f0[EPT], f1[EPT];
+	const size_t sbase = size_t(i) * (M + 1) * N;
+	const bool ha = specb != nullptr;
+	cpx<T> f0[EPT], f1[EPT], g0[EPT], g1[EPT];
 #pragma unroll
 	for (int e = 0; e < EPT; ++e) {
 		const int q = tid + e * kThreads;
-		f0[e] = f1[e] = cpx<T>{T(0), T(0)};
+		f0[e] = f1[e] = g0[e] = g1[e] = cpx<T>{T(0), T(0)};
 		if (q < nel) {
 			const int c = q / N, kx = q % N;
-			f0[e] = base[size_t(c0 + c) * N + kx];
-			if (c0 + c == 0) f1[e] = base[size_t(M) * N + kx];
+			f0[e] = spec[sbase + size_t(c0 + c) * N + kx];
+			if (c0 + c == 0) f1[e] = spec[sbase + size_t(M) * N + kx];
+			if (ha) {
+				g0[e] = specb[sbase + size_t(c0 + c) * N + kx];
+				if (c0 + c == 0) g1[e] = specb[sbase + size_t(M) * N + kx];
+			}
 		}
 	}
 	const double wx = 1.0 / kmin[4 * i + 0];
@@ -499,8 +577,15 @@ __device__ __forceinline__ void col_task(cpx<T>* X, int ny, int M, int nz, int i
 					wb.init(rot_index(kxm, N), rot_index(ky == 0 ? 0 : ny - ky, ny), du, dv, zs,
 					        dz, h, zb);
 					for (int zz = 0; zz < zb; ++zz) {
-						const T w = T(0.5 * (wa.next(zz, zs, dz, h) + wb.next(zz, zs, dz, h)));
-						dst[zz * CB * N] = cpx<T>{f0[e].x * w, f0[e].y * w};
+						const double va = wa.next(zz, zs, dz, h), vb = wb.next(zz, zs, dz, h);
+						const T w = T(0.5 * (va + vb));
+						if (ha) {
+							const T wd = T(0.5 * (va - vb));
+							dst[zz * CB * N] = cpx<T>{f0[e].x * w - g0[e].y * wd,
+							                          f0[e].y * w + g0[e].x * wd};
+						} else {
+							dst[zz * CB * N] = cpx<T>{f0[e].x * w, f0[e].y * w};
+						}
 					}
 				}
 				if (ky == 0) {
@@ -509,9 +594,16 @@ __device__ __forceinline__ void col_task(cpx<T>* X, int ny, int M, int nz, int i
 					wa.init(rot_index(kx, N), rot_index(M, ny), du, dv, zs, dz, h, zb);
 					wb.init(rot_index(kxm, N), rot_index(M, ny), du, dv, zs, dz, h, zb);
 					for (int zz = 0; zz < zb; ++zz) {
-						const T w = T(0.5 * (wa.next(zz, zs, dz, h) + wb.next(zz, zs, dz, h)));
+						const double va = wa.next(zz, zs, dz, h), vb = wb.next(zz, zs, dz, h);
+						const T w = T(0.5 * (va + vb));
 						const cpx<T> a = dst[zz * CB * N];
-						dst[zz * CB * N] = cpx<T>{a.x - f1[e].y * w, a.y + f1[e].x * w};
+						if (ha) {
+							const T wd = T(0.5 * (va - vb));
+							const cpx<T> hm{f1[e].x * w - g1[e].y * wd, f1[e].y * w + g1[e].x * wd};
+							dst[zz * CB * N] = cpx<T>{a.x - hm.y, a.y + hm.x};
+						} else {
+							dst[zz * CB * N] = cpx<T>{a.x - f1[e].y * w, a.y + f1[e].x * w};
+						}
 					}
 				}
 			}
@@ -556,33 +648,44 @@ __device__ __forceinline__ void row_task(cpx<T>* X, int nx, int ny, int x0,
                                          T* __restrict__ out, int R) {
 	constexpr int M = N1 * N2;
 	const int tid = threadIdx.x;
-	if (tid < R * N1) {
-		const int r = tid % R, n1 = tid / R;
-		// rows past nx (none for the launch shapes used) transform a copy of
-		// the last row and are not stored
-		const int x = min(x0 + r, nx - 1);
+	const bool act = tid < R * N1;
+	const int r = tid % R, n1 = tid / R;
+	// rows past nx (none for the launch shapes used) transform a copy of the
+	// last row and are not stored
+	const int x = min(x0 + r, nx - 1);
+	cpx<T> hk[N2];
+	double2 tk[N2];
+	if (act) {
 		// every load is issued before any arithmetic, with no branch between
 		// them, so that all of them are in flight at once
-		cpx<T> hk[N2], hm[N2];
-		double2 tk[N2];
 #pragma unroll
 		for (int n2 = 0; n2 < N2; ++n2) {
 			const int k = n1 + N1 * n2;
 			hk[n2] = g[size_t(k) * nx + x];
-			hm[n2] = g[size_t(k == 0 ? 0 : M - k) * nx + x];
 			tk[n2] = twy[k];
 		}
-		cpx<T> v[N2];
+		// the rows are staged in LDS: the partner H[M-k] of a lane's H[k] is
+		// held by another lane and is read from there, not from memory again
+#pragma unroll
+		for (int n2 = 0; n2 < N2; ++n2) X[r * (M + 1) + n1 + N1 * n2] = hk[n2];
+	}
+	__syncthreads();
+	cpx<T> v[N2];
+	if (act) {
 #pragma unroll
 		for (int n2 = 0; n2 < N2; ++n2) {
+			const int k = n1 + N1 * n2;
 			// Z[k] = (H[k] + conj H[M-k]) + i e^{i pi k/M} (H[k] - conj H[M-k])
-			const cpx<T> a = hk[n2], b = conj(hm[n2]);
+			const cpx<T> a = hk[n2], b = conj(X[r * (M + 1) + (k == 0 ? 0 : M - k)]);
 			const cpx<T> sum = a + b;
 			const cpx<T> wd = cmul(a - b, T(tk[n2].x), T(-tk[n2].y));
 			v[n2] = cpx<T>{sum.x - wd.y, sum.y + wd.x};
 		}
 		// packed column k = 0: H[0] = real part, H[M] = imaginary part
 		if (n1 == 0) v[0] = cpx<T>{hk[0].x + hk[0].y, hk[0].x - hk[0].y};
+	}
+	__syncthreads();  // the exchange below overwrites the staged rows
+	if (act) {
 		dft<N2, +1>(v);
 		twiddle_out<N2, +1>(v, twy, 2 * n1,
 		                    [&](int k2, cpx<T> a) { X[xidx<N1, N2>(r, k2, n1)] = a; });
@@ -612,13 +715,14 @@ __device__ __forceinline__ void row_task(cpx<T>* X, int nx, int ny, int x0,
 template <class T, int N1, int N2>
 __global__ __launch_bounds__(kThreads) void inv_cols_kernel(
 	int ny, int M, int nz, int i0, const double2* __restrict__ twx,
-	const cpx<T>* __restrict__ spec, const double* __restrict__ kmin, double z0, double dz,
-	double h, cpx<T>* __restrict__ G, int CB, int ZB) {
+	const cpx<T>* __restrict__ spec, const cpx<T>* __restrict__ specb,
+	const double* __restrict__ kmin, double z0, double dz, double h, cpx<T>* __restrict__ G,
+	int CB, int ZB) {
 	extern __shared__ unsigned char smem_raw[];
 	cpx<T>* X = reinterpret_cast<cpx<T>*>(smem_raw);
 	const int il = blockIdx.y;
-	col_task<T, N1, N2>(X, ny, M, nz, i0 + il, blockIdx.x * CB, twx, spec, kmin, z0, dz, h,
-	                    G + size_t(il) * nz * M * (N1 * N2), CB, ZB);
+	col_task<T, N1, N2>(X, ny, M, nz, i0 + il, blockIdx.x * CB, twx, spec, specb, kmin, z0, dz,
+	                    h, G + size_t(il) * nz * M * (N1 * N2), CB, ZB);
 }
 
 // Row pass over a chunk: block (rb, item) runs the row task of
@@ -687,6 +791,9 @@ void allow_lds(K kernel, size_t lds) {
 struct PotPlan {
 	int nt, nx, ny, M, nto, nz;
 	double dt, z0, dz, h;
+	double dx, dy;    // grid spacing along x and y (derivatives of the high-amplitude solver)
+	int solver;       // 0 linear, 1 high amplitude
+	int nfields() const { return solver == 1 ? 2 : 1; }
 };
 
 // Length-dependent launches, instantiated for every power of two.
@@ -698,8 +805,9 @@ struct ColLaunch {
 		const int S = std::min(p.M + 1, kThreads / N1);
 		const size_t lds = sizeof(cpx<T>) * size_t(xsize<N1, N2>(S));
 		allow_lds(&fwd_cols_kernel<T, N1, N2>, lds);
-		fwd_cols_kernel<T, N1, N2><<<dim3(div_ceil(p.M + 1, S), p.nto), kThreads, lds,
-		                             ctx->stream>>>(NX, p.M, twx, spec, S);
+		// the half spectra of every field are contiguous: one launch covers them
+		fwd_cols_kernel<T, N1, N2><<<dim3(div_ceil(p.M + 1, S), p.nto * p.nfields()), kThreads,
+		                             lds, ctx->stream>>>(NX, p.M, twx, spec, S);
 		ARMA_HIP_CHECK(hipGetLastError());
 	}
 
@@ -712,14 +820,21 @@ struct RowLaunch {
 	static int rows(const PotPlan& p) { return std::min(p.nx, kThreads / N1); }
 
 	static void forward(arma_hip_ctx* ctx, const PotPlan& p, const T* zeta, const int* didx,
-	                    const double2* twy, cpx<T>* spec) {
+	                    const double2* twy, cpx<T>* spec, const T* field) {
 		const int R = rows(p);
 		const size_t lds = sizeof(cpx<T>) * size_t(std::max(xsize<N1, N2>(R), R * (M + 1)));
 		allow_lds(&fwd_rows_kernel<T, N1, N2>, lds);
-		fwd_rows_kernel<T, N1, N2><<<dim3(div_ceil(p.nx, R), p.nto), kThreads, lds,
-		                             ctx->stream>>>(zeta, p.nt, p.nx, p.ny, didx, p.dt, twy,
-		                                            spec, R);
-		ARMA_HIP_CHECK(hipGetLastError());
+		// linear: -zeta_t; high amplitude: the planes Re F, then Im F, into
+		// consecutive blocks of spec
+		const size_t sblock = size_t(p.nto) * (M + 1) * p.nx;
+		const size_t fblock = size_t(p.nto) * p.nx * p.ny;
+		for (int f = 0; f < p.nfields(); ++f) {
+			fwd_rows_kernel<T, N1, N2><<<dim3(div_ceil(p.nx, R), p.nto), kThreads, lds,
+			                             ctx->stream>>>(
+				zeta, p.nt, p.nx, p.ny, didx, p.dt, field ? field + f * fblock : nullptr, twy,
+				spec + f * sblock, R);
+			ARMA_HIP_CHECK(hipGetLastError());
+		}
 	}
 
 };
@@ -758,7 +873,9 @@ void launch_inverse(arma_hip_ctx* ctx, const PotPlan& p, const double2* twx, con
 		{
 			KernelTimer t(ctx, "pot_inv_cols");
 			inv_cols_kernel<T, CN1, CN2><<<dim3(M / CB, cn), kThreads, lds_c, ctx->stream>>>(
-				p.ny, M, p.nz, i0, twx, spec, dk, p.z0, p.dz, p.h, G, CB, ZB);
+				p.ny, M, p.nz, i0, twx, spec,
+				p.solver == 1 ? spec + size_t(p.nto) * (M + 1) * NX : nullptr, dk, p.z0, p.dz, p.h,
+				G, CB, ZB);
 			ARMA_HIP_CHECK(hipGetLastError());
 		}
 		{
@@ -823,16 +940,19 @@ struct Ops {
 	template <int L>
 	struct RowFwd {
 		static void run(arma_hip_ctx* c, const PotPlan& p, const T* z, const int* di,
-		                const double2* tw, cpx<T>* s) {
-			if constexpr (L <= 512) RowLaunch<T, L>::forward(c, p, z, di, tw, s);
+		                const double2* tw, cpx<T>* s, const T* field) {
+			if constexpr (L <= 512) RowLaunch<T, L>::forward(c, p, z, di, tw, s, field);
 		}
 	};
 };
 
 template <class T>
 void run_potential(arma_hip_ctx* ctx, const arma_pot_desc* d, const T* zeta, T* phi,
-                   double* wnmax_out) {
+                   double* wnmax_out, int solver) {
 	PotPlan p;
+	p.solver = solver;
+	p.dx = d->delta[1];
+	p.dy = d->delta[2];
 	p.nt = d->shape[0];
 	p.nx = d->shape[1];
 	p.ny = d->shape[2];
@@ -879,12 +999,22 @@ void run_potential(arma_hip_ctx* ctx, const arma_pot_desc* d, const T* zeta, T* 
 		                                                       d->delta[0], dk);
 		ARMA_HIP_CHECK(hipGetLastError());
 	}
-	// forward half spectra of every output slice: spec[i][k][kx], k = 0..M
-	cpx<T>* spec = (cpx<T>*)ctx->pot_spec.get(sizeof(cpx<T>) * size_t(p.nto) * (p.M + 1) * p.nx);
+	// forward half spectra of every output slice: spec[f][i][k][kx], k = 0..M,
+	// one field f for the linear solver, two (A, B) for the high-amplitude one
+	cpx<T>* spec = (cpx<T>*)ctx->pot_spec.get(sizeof(cpx<T>) * size_t(p.nfields()) * p.nto *
+	                                          (p.M + 1) * p.nx);
 	{
 		KernelTimer t_fwd(ctx, "pot_forward");
+		T* field = nullptr;
+		if (p.solver == 1) {
+			const size_t slice = size_t(p.nx) * p.ny;
+			field = (T*)ctx->pot_field.get(sizeof(T) * 2 * p.nto * slice);
+			ha_field_kernel<T><<<dim3(div_ceil(int(slice), 256), p.nto), 256, 0, st>>>(
+				zeta, p.nt, p.nx, p.ny, p.nto, didx, p.dt, p.dx, p.dy, field);
+			ARMA_HIP_CHECK(hipGetLastError());
+		}
 		by_length<Ops<T>::template RowFwd>(p.M, ctx, p, zeta, (const int*)didx,
-		                                   (const double2*)twy, spec);
+		                                   (const double2*)twy, spec, (const T*)field);
 		by_length<Ops<T>::template ColFwd>(p.nx, ctx, p, (const double2*)twx, spec);
 	}
 	// every (slice, level): window, inverse along x, inverse along y
@@ -909,12 +1039,17 @@ void run_potential(arma_hip_ctx* ctx, const arma_pot_desc* d, const T* zeta, T* 
 
 }  // namespace
 
+void launch_potential(arma_hip_ctx* ctx, const arma_pot_desc* d, const void* zeta, void* phi,
+                      double* wnmax_out, int solver) {
+	if (d->precision == ARMA_FP32)
+		run_potential<float>(ctx, d, (const float*)zeta, (float*)phi, wnmax_out, solver);
+	else
+		run_potential<double>(ctx, d, (const double*)zeta, (double*)phi, wnmax_out, solver);
+}
+
 void launch_linear_potential(arma_hip_ctx* ctx, const arma_pot_desc* d, const void* zeta,
                              void* phi, double* wnmax_out) {
-	if (d->precision == ARMA_FP32)
-		run_potential<float>(ctx, d, (const float*)zeta, (float*)phi, wnmax_out);
-	else
-		run_potential<double>(ctx, d, (const double*)zeta, (double*)phi, wnmax_out);
+	launch_potential(ctx, d, zeta, phi, wnmax_out, 0);
 }
 
 }  // namespace arma_hip

@@ -400,7 +400,8 @@ Array4D<T> Linear_solver<T>::operator()(const Array3D<T>& zeta, const Grid<3>& g
 	}
 	d.precision = sizeof(T) == 4 ? ARMA_FP32 : ARMA_FP64;
 	Array4D<T> out({this->domain_.npoints[0], this->domain_.npoints[1], zeta.shape[1], zeta.shape[2]});
-	check(arma_hip_linear_potential(ctx_->get(), &d, zeta.data.data(), out.data.data(), 0, nullptr));
+	auto entry = high_amplitude() ? arma_hip_high_amplitude_potential : arma_hip_linear_potential;
+	check(entry(ctx_->get(), &d, zeta.data.data(), out.data.data(), 0, nullptr));
 	return out;
 }
 
@@ -530,6 +531,8 @@ template class velocity::Velocity_potential_solver<double>;
 template class velocity::Velocity_potential_solver<float>;
 template class velocity::Linear_solver<double>;
 template class velocity::Linear_solver<float>;
+template class velocity::High_amplitude_solver<double>;
+template class velocity::High_amplitude_solver<float>;
 template class ARMA_driver<double>;
 template class ARMA_driver<float>;
 

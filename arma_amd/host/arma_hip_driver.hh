@@ -1,7 +1,8 @@
 // C++ drop-in for the reference's generator/solver registry and driver API
-// with the HIP framework: AR_model<T> and Linear_solver<T> keep their
-// registry keys ("AR", "linear") and .arma input syntax, and their compute
-// bodies call libarma_hip through the C ABI (include/arma_hip.h).
+// with the HIP framework: AR_model<T>, Linear_solver<T> and
+// High_amplitude_solver<T> keep their registry keys ("AR", "linear",
+// "high_amplitude") and .arma input syntax, and their compute bodies call
+// libarma_hip through the C ABI (include/arma_hip.h).
 //
 // Reference interfaces mirrored here:
 //   ARMA_driver<T> ............ src/arma_driver.hh:43-150, src/arma_driver.cc
@@ -12,6 +13,7 @@
 //   AR_model<T> ............... src/generator/ar_model.hh:27-140
 //   Basic_ARMA_model::generate  src/generator/basic_arma_model.cc:51-113
 //   Linear_solver<T> .......... src/velocity/linear_solver.hh:18-70
+//   High_amplitude_solver<T> .. src/velocity/high_amplitude_solver.hh:12-45
 //   Velocity_potential_solver . src/velocity/basic_solver.hh:32-129
 //   Output_flags .............. src/output_flags.cc:27-87
 //   Binary_stream (zeta.bin) .. src/io/binary_stream.hh:19-71
@@ -190,8 +192,23 @@ public:
 	explicit Linear_solver(std::shared_ptr<Hip_context> ctx) : ctx_(std::move(ctx)) {}
 	Array4D<T> operator()(const Array3D<T>& zeta, const Grid<3>& grid) override;
 
+protected:
+	// the multiplier whose transform is the potential: -zeta_t here
+	virtual bool high_amplitude() const { return false; }
+
 private:
 	std::shared_ptr<Hip_context> ctx_;
+};
+
+// The same transforms of zeta_t / (i((zeta_x + zeta_y)/s - zeta_x - zeta_y) - 1/s)
+// (src/velocity/high_amplitude_solver.cc:6-23).
+template <class T>
+class High_amplitude_solver : public Linear_solver<T> {
+public:
+	using Linear_solver<T>::Linear_solver;
+
+protected:
+	bool high_amplitude() const override { return true; }
 };
 
 }  // namespace velocity
@@ -241,6 +258,7 @@ template <class T>
 void register_all(ARMA_driver<T>& drv) {
 	drv.template register_model<generator::AR_model<T>>("AR");
 	drv.template register_solver<velocity::Linear_solver<T>>("linear");
+	drv.template register_solver<velocity::High_amplitude_solver<T>>("high_amplitude");
 }
 
 }  // namespace arma

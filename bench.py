@@ -37,6 +37,12 @@ WORKLOADS = {
         acf=("propagating", 0.8, (-0.8, 0.0)), shape=(1024, 256, 256), order=7,
         partition=(16, 32, 32), seed=0, depth=12.0, lbound=(0.0, -10.0),
         ubound=(1023.0, 0.0), npoints=(1024, 16), precision="fp64"),
+    # the same surface under the high-amplitude potential solver
+    # (velocity_potential_solver = high_amplitude)
+    "ar-potential-ha-1024x256x256-fp64": dict(
+        acf=("propagating", 0.8, (-0.8, 0.0)), shape=(1024, 256, 256), order=7,
+        partition=(16, 32, 32), seed=0, depth=12.0, lbound=(0.0, -10.0),
+        ubound=(1023.0, 0.0), npoints=(1024, 16), precision="fp64", solver="high_amplitude"),
     # precision variant: FP32 storage, compensated accumulation, FP64 setup
     "ar-potential-2048x512x512-fp32": dict(
         acf=("propagating", 0.8, (-0.8, 0.0)), shape=(2048, 512, 512), order=7,
@@ -421,7 +427,8 @@ def cpu_baseline(w, phi, varwn, params, seeds, gpu_sample):
         nz = w["npoints"][1]
         t0 = time.perf_counter()
         pot = O.linear_potential(zeta, (2 * math.pi,) * 3, w["depth"], (0.0, w["lbound"][1]),
-                                 (float(tp - 1), w["ubound"][1]), (tp, nz))
+                                 (float(tp - 1), w["ubound"][1]), (tp, nz),
+                                 solver=w.get("solver", "linear"))
         t_pot = time.perf_counter() - t0
         res["potential_pts_per_s"] = tp * nz * nx * ny / t_pot
         desc += f"; potential of {tp} time slices x {nz} z levels"
@@ -511,6 +518,7 @@ def main():
                        algorithm=algo)
     zeta = torch.empty(w["shape"], dtype=tdt, device="cuda:0")
     has_pot = w.get("depth") is not None
+    solver = w.get("solver", "linear")
     if has_pot:
         nz = w["npoints"][1]
         pdesc = ctx.pot_desc(w["shape"], (2 * math.pi,) * 3, w["depth"], w["lbound"],
@@ -532,7 +540,7 @@ def main():
         if has_pot:
             if timed:
                 ev["potential"][0].record()
-            ctx.linear_potential(pdesc, zeta, phi_out)
+            ctx.linear_potential(pdesc, zeta, phi_out, solver=solver)
             if timed:
                 ev["potential"][1].record()
 
@@ -576,7 +584,8 @@ def main():
         elif name == "noise_partitioned" or name == "noise_sequential":
             entry.update(bound="hbm", unit="GB/s", achieved=surf_pts * esize / mean_s / 1e9,
                          peak=HBM_PEAK_GBS)
-        elif name == "pot_inv_rows":
+        elif name in ("pot_inv_rows", "pot_inv_cols"):
+            # each pass of the 2-D inverse produces one phi-sized array per chunk
             chunk_pts = pot_pts / cnt * args.steps
             entry.update(bound="hbm", unit="GB/s", achieved=chunk_pts * esize / mean_s / 1e9,
                          peak=HBM_PEAK_GBS)
@@ -603,7 +612,7 @@ def main():
             sdesc = ctx.pot_desc(w["shape"], (2 * math.pi,) * 3, w["depth"],
                                  (0.0, w["lbound"][1]), (float(tp - 1), w["ubound"][1]),
                                  (tp, nz), prec)
-            ctx.linear_potential(sdesc, zeta, phi_out)
+            ctx.linear_potential(sdesc, zeta, phi_out, solver=solver)
             gpu_sample["potential"] = phi_out[:tp].double().cpu().numpy()
         log("bench: timing the CPU oracle on a sample")
         try:
@@ -621,6 +630,7 @@ def main():
                    "phi_shape": list(phi.shape), "partition": list(part),
                    "noise_streams": nstreams, "ar_kernel": args.ar,
                    "potential": list(w["npoints"]) + [nx, ny] if has_pot else None,
+                   "solver": solver if has_pot else None,
                    "setup": "FP64 Yule-Walker (Choi) and white-noise variance on the GPU",
                    "parallelism": "single-gpu"},
         "surface_pts_per_s": surf_rate, "potential_pts_per_s": pot_rate,

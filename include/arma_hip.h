@@ -148,6 +148,17 @@ int arma_hip_rescale(arma_hip_ctx* ctx, void* zeta, const int32_t shape[3], int3
 int arma_hip_linear_potential(arma_hip_ctx* ctx, const arma_pot_desc* desc, const void* zeta,
                               void* phi, int on_device, double* wnmax_out);
 
+/* The same under the high-amplitude solver: replaces
+   High_amplitude_solver<T>::operator() (registered as "high_amplitude",
+   src/register_all.hh:53; precompute src/velocity/high_amplitude_solver.cc:6-23,
+   transforms src/velocity/linear_solver.cc:54-121), whose multiplier is
+   zeta_t / (i((zeta_x + zeta_y)/s - zeta_x - zeta_y) - 1/s),
+   s = sqrt(1 + zeta_x^2 + zeta_y^2), instead of -zeta_t.  Same arguments,
+   layout and errors as arma_hip_linear_potential. */
+int arma_hip_high_amplitude_potential(arma_hip_ctx* ctx, const arma_pot_desc* desc,
+                                      const void* zeta, void* phi, int on_device,
+                                      double* wnmax_out);
+
 /* Raw tempered uint32 draws of n_streams streams (draws_per_stream each)
    into out (device), stream-major: the bit-exact check of the generator. */
 int arma_hip_mt_raw(arma_hip_ctx* ctx, const arma_mt_params* mts, const uint32_t* seeds,

@@ -35,6 +35,18 @@
  *                       rotate index map.
  *   orc_linear_potential pinned: single-Fourier-mode analytic solution and
  *                       numpy restatement on 16x16 grids.
+ *   orc_derivative_xy   pinned: exact on quadratics in x and y, raw slice for
+ *                       extents below 3.
+ *   orc_high_amplitude_potential
+ *                       pinned: closed form for a plane surface
+ *                       zeta = a x + b y + c t (phi = 4 pi Re(c / D) with the
+ *                       constant denominator D), the small-slope limit equal
+ *                       to the linear solver to O(slope^2), and a numpy
+ *                       restatement with complex FFTs.  The sign of Im D
+ *                       enters only through the odd part of the rotated
+ *                       window; nothing but the restatement of
+ *                       src/velocity/high_amplitude_solver.cc:20-22 fixes it
+ *                       (parity unpinned for that sign).
  */
 #include "arma_oracle.h"
 
@@ -1018,20 +1030,69 @@ void orc_interp_edges(double* res, int nx, int ny) {
 		res[l] = interp3(res, ny, 1, l - 1, 1, l, 2, l - 1, 0, l);
 }
 
+/* src/derivative.hh:8-73 along x (dim 1) or y (dim 2) at slice idx_t:
+   central differences inside, second-order one-sided at the borders, the raw
+   slice when the extent is below 3. */
+void orc_derivative_xy(const double* zeta, const int shape[3], double d, int dim, int idx_t,
+                       double* out) {
+	const int nx = shape[1], ny = shape[2];
+	const size_t s = (size_t)nx * ny;
+	const double* f = zeta + (size_t)idx_t * s;
+	const int n = dim == 1 ? nx : ny;
+	const size_t step = dim == 1 ? (size_t)ny : 1;
+	if (n - 1 < 2) {
+		memcpy(out, f, sizeof(double) * s);
+		return;
+	}
+	for (int i = 0; i < nx; ++i) {
+		for (int j = 0; j < ny; ++j) {
+			const size_t q = (size_t)i * ny + j;
+			const int c = dim == 1 ? i : j;
+			double r;
+			if (c == 0) {
+				r = 0.5 * (-f[q + 2 * step] + 4.0 * f[q + step] - 3.0 * f[q]) / d;
+			} else if (c == n - 1) {
+				r = 0.5 * (3.0 * f[q] - 4.0 * f[q - step] + f[q - 2 * step]) / d;
+			} else {
+				r = 0.5 * (f[q + step] - f[q - step]) / d;
+			}
+			out[q] = r;
+		}
+	}
+}
+
+/* The multiplier of High_amplitude_solver::precompute
+   (src/velocity/high_amplitude_solver.cc:14-22) at one point:
+   F = zeta_t / (i ((zx + zy)/s - zx - zy) - 1/s),  s = sqrt(1 + zx^2 + zy^2),
+   with zeta_t = +d zeta/dt.  The division is C's complex division (the same
+   library routine std::complex<double> division calls under g++). */
+static void ha_multiplier(double zt, double zx, double zy, double* re, double* im) {
+	const double s = sqrt(1.0 + zx * zx + zy * zy);
+	const double e = (zx + zy) / s - zx - zy;
+	const _Complex double den = __builtin_complex(0.0 - 1.0 / s, e);
+	const _Complex double f = __builtin_complex(zt, 0.0) / den;
+	*re = __real__ f;
+	*im = __imag__ f;
+}
+
 /* Velocity_potential_solver::operator() (src/velocity/basic_solver.cc:39-77)
-   with Linear_solver (src/velocity/linear_solver.cc:20-121) and the wave
-   number range of src/velocity/basic_solver.cc:95-112.  Output layout
-   (nt_domain, nz, nx, ny) row-major.  wnmax_out (optional) receives the
-   per-slice (1/kmin_x, 1/kmin_y). */
-int orc_linear_potential(const double* zeta, const int shape[3], const double delta[3],
-                         double depth, const double lbound[2], const double ubound[2],
-                         const int npoints[2], double* out, double* wnmax_out) {
+   with Linear_solver (src/velocity/linear_solver.cc:20-121) or, for
+   solver 1, High_amplitude_solver (src/velocity/high_amplitude_solver.cc:6-23,
+   the same transforms of a complex multiplier), and the wave number range of
+   src/velocity/basic_solver.cc:95-112.  Output layout (nt_domain, nz, nx, ny)
+   row-major.  wnmax_out (optional) receives the per-slice
+   (1/kmin_x, 1/kmin_y). */
+static int potential(const double* zeta, const int shape[3], const double delta[3],
+                     double depth, const double lbound[2], const double ubound[2],
+                     const int npoints[2], double* out, double* wnmax_out, int solver) {
 	const int nx = shape[1], ny = shape[2];
 	const size_t s = (size_t)nx * ny;
 	if (lbound[1] < -depth) return -1;
 	const double dtd = (ubound[0] - lbound[0]) / (double)(npoints[0] - 1 > 1 ? npoints[0] - 1 : 1);
 	const double dzd = (ubound[1] - lbound[1]) / (double)(npoints[1] - 1 > 1 ? npoints[1] - 1 : 1);
 	double* zt = (double*)malloc(sizeof(double) * s);
+	double* zx = (double*)malloc(sizeof(double) * s);
+	double* zy = (double*)malloc(sizeof(double) * s);
 	double* fr = (double*)malloc(sizeof(double) * s);
 	double* fi = (double*)malloc(sizeof(double) * s);
 	double* wr = (double*)malloc(sizeof(double) * s);
@@ -1041,7 +1102,13 @@ int orc_linear_potential(const double* zeta, const int shape[3], const double de
 		const double tval = lbound[0] + dtd * i;
 		const int idx_t = (int)tval;
 		orc_derivative_t(zeta, shape, delta[0], idx_t, zt);
-		for (size_t q = 0; q < s; ++q) { fr[q] = -zt[q]; fi[q] = 0.0; }
+		if (solver == 1) {
+			orc_derivative_xy(zeta, shape, delta[1], 1, idx_t, zx);
+			orc_derivative_xy(zeta, shape, delta[2], 2, idx_t, zy);
+			for (size_t q = 0; q < s; ++q) ha_multiplier(zt[q], zx[q], zy[q], fr + q, fi + q);
+		} else {
+			for (size_t q = 0; q < s; ++q) { fr[q] = -zt[q]; fi[q] = 0.0; }
+		}
 		double kr[4];
 		const double tgrid = delta[0] * idx_t;
 		orc_factor_waves(zeta + (size_t)idx_t * s, nx, ny, (int)tgrid, delta[0], kr);
@@ -1058,6 +1125,18 @@ int orc_linear_potential(const double* zeta, const int shape[3], const double de
 			orc_interp_edges(res, nx, ny);
 		}
 	}
-	free(zt); free(fr); free(fi); free(wr); free(wi); free(win);
+	free(zt); free(zx); free(zy); free(fr); free(fi); free(wr); free(wi); free(win);
 	return 0;
+}
+
+int orc_linear_potential(const double* zeta, const int shape[3], const double delta[3],
+                         double depth, const double lbound[2], const double ubound[2],
+                         const int npoints[2], double* out, double* wnmax_out) {
+	return potential(zeta, shape, delta, depth, lbound, ubound, npoints, out, wnmax_out, 0);
+}
+
+int orc_high_amplitude_potential(const double* zeta, const int shape[3], const double delta[3],
+                                 double depth, const double lbound[2], const double ubound[2],
+                                 const int npoints[2], double* out, double* wnmax_out) {
+	return potential(zeta, shape, delta, depth, lbound, ubound, npoints, out, wnmax_out, 1);
 }

@@ -109,6 +109,14 @@ int orc_linear_potential(const double* zeta, const int shape[3], const double de
                          double depth, const double lbound[2], const double ubound[2],
                          const int npoints[2], double* out, double* wnmax_out);
 
+/* ---- velocity potential (high-amplitude solver) ---- */
+
+void orc_derivative_xy(const double* zeta, const int shape[3], double d, int dim, int idx_t,
+                       double* out);
+int orc_high_amplitude_potential(const double* zeta, const int shape[3], const double delta[3],
+                                 double depth, const double lbound[2], const double ubound[2],
+                                 const int npoints[2], double* out, double* wnmax_out);
+
 #ifdef __cplusplus
 }
 #endif

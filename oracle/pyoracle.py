@@ -80,6 +80,8 @@ def lib():
         L.orc_interp_edges.argtypes = [dp, C.c_int, C.c_int]
         L.orc_linear_potential.argtypes = [dp, ip3, C.c_double * 3, C.c_double, C.c_double * 2,
                                            C.c_double * 2, C.c_int * 2, dp, C.c_void_p]
+        L.orc_derivative_xy.argtypes = [dp, ip3, C.c_double, C.c_int, C.c_int, dp]
+        L.orc_high_amplitude_potential.argtypes = L.orc_linear_potential.argtypes
         _lib = L
     return _lib
 
@@ -242,15 +244,31 @@ def interp_edges(res):
     return r
 
 
-def linear_potential(zeta, delta, depth, lbound, ubound, npoints, return_wnmax=False):
+def derivative_xy(zeta, d, dim, idx_t):
+    """d zeta / dx (dim 1) or d zeta / dy (dim 2) at slice idx_t."""
+    z = np.ascontiguousarray(zeta, np.float64)
+    out = np.empty(z.shape[1:], np.float64)
+    lib().orc_derivative_xy(z.reshape(-1), _i3(z.shape), d, dim, idx_t, out.reshape(-1))
+    return out
+
+
+def linear_potential(zeta, delta, depth, lbound, ubound, npoints, return_wnmax=False,
+                     solver="linear"):
     z = np.ascontiguousarray(zeta, np.float64)
     out = np.empty((npoints[0], npoints[1], z.shape[1], z.shape[2]), np.float64)
     wn = np.empty(2 * npoints[0], np.float64)
-    rc = lib().orc_linear_potential(z.reshape(-1), _i3(z.shape), (C.c_double * 3)(*delta), depth,
-                                    (C.c_double * 2)(*lbound), (C.c_double * 2)(*ubound),
-                                    (C.c_int * 2)(*npoints), out.reshape(-1), wn.ctypes.data)
+    fn = {"linear": lib().orc_linear_potential,
+          "high_amplitude": lib().orc_high_amplitude_potential}[solver]
+    rc = fn(z.reshape(-1), _i3(z.shape), (C.c_double * 3)(*delta), depth,
+            (C.c_double * 2)(*lbound), (C.c_double * 2)(*ubound),
+            (C.c_int * 2)(*npoints), out.reshape(-1), wn.ctypes.data)
     if rc != 0:
         raise ValueError("min(z) is less than -h")
     if return_wnmax:
         return out, wn.reshape(-1, 2)
     return out
+
+
+def high_amplitude_potential(zeta, delta, depth, lbound, ubound, npoints, return_wnmax=False):
+    return linear_potential(zeta, delta, depth, lbound, ubound, npoints, return_wnmax,
+                            solver="high_amplitude")

@@ -1,5 +1,5 @@
 """The C++ drop-in driver (arma_amd/host/arma-hip): the reference's .arma
-input syntax, registry keys "AR"/"linear", model setup, and (on a GPU) the
+input syntax, registry keys "AR"/"linear"/"high_amplitude", model setup, and (on a GPU) the
 whole generate -> potential -> write path against the oracle.
 """
 import os
@@ -27,16 +27,16 @@ model = AR {
 	output = surface,%(fmt)s
 	validate = 0
 }
-velocity_potential_solver = linear {
+velocity_potential_solver = %(solver)s {
 	depth = 12
 	domain = from (0,-10) to (%(tlast)d,0) npoints (%(nt)d,%(nz)d)
 }
 """
 
 
-def write_input(tmp_path, nt=40, nx=32, ny=32, nz=4, fmt="binary"):
+def write_input(tmp_path, nt=40, nx=32, ny=32, nz=4, fmt="binary", solver="linear"):
     p = tmp_path / "in.arma"
-    p.write_text(INPUT % dict(nt=nt, nx=nx, ny=ny, nz=nz, fmt=fmt, tlast=nt - 1))
+    p.write_text(INPUT % dict(nt=nt, nx=nx, ny=ny, nz=nz, fmt=fmt, tlast=nt - 1, solver=solver))
     return str(p)
 
 
@@ -80,13 +80,21 @@ def test_input_errors(tmp_path, bad, msg):
     assert r.returncode != 0 and msg in r.stderr
 
 
+def test_unknown_solver(tmp_path):
+    text = open(write_input(tmp_path)).read().replace("= linear {", "= small_amplitude {")
+    (tmp_path / "in.arma").write_text(text)
+    r = run(["-n", str(tmp_path / "in.arma")], str(tmp_path))
+    assert r.returncode != 0 and "Invalid object" in r.stderr
+
+
 @pytest.mark.gpu
-def test_end_to_end_surface_and_potential(tmp_path):
+@pytest.mark.parametrize("solver", ["linear", "high_amplitude"])
+def test_end_to_end_surface_and_potential(tmp_path, solver):
     """arma-hip on a 40x32x32 standing-wave case; zeta.bin is big-endian."""
     import arma_amd
     arma_amd.write_mt_config(str(tmp_path / "mt.dat"),
                              arma_amd._abi.params_array([arma_amd.mt19937_params()]))
-    r = run([write_input(tmp_path, fmt="binary,csv")], str(tmp_path))
+    r = run([write_input(tmp_path, fmt="binary,csv", solver=solver)], str(tmp_path))
     assert r.returncode == 0, r.stderr
     zeta = np.fromfile(tmp_path / "zeta.bin", dtype=">f8").reshape(40, 32, 32)
     acf = O.acf(O.STANDING, (10, 10, 10), velocity=0.8, beta=(0.8, 0.8))
@@ -99,6 +107,7 @@ def test_end_to_end_surface_and_potential(tmp_path):
     assert err < 1e-10, err
     pot = np.loadtxt(tmp_path / "phi.csv", delimiter=",", skiprows=1)[:, 4].reshape(40, 4, 32, 32)
     delta = (2 * np.pi,) * 3
-    pref = O.linear_potential(zeta, delta, 12.0, (0.0, -10.0), (39.0, 0.0), (40, 4))
+    pref = O.linear_potential(zeta, delta, 12.0, (0.0, -10.0), (39.0, 0.0), (40, 4),
+                              solver=solver)
     perr = float(np.max(np.abs(pot - pref)) / np.max(np.abs(pref)))
     assert perr < 1e-9, perr

@@ -1,4 +1,5 @@
-"""GPU parity of the linear velocity potential, through the C ABI.
+"""GPU parity of the linear and high-amplitude velocity potentials, through
+the C ABI.
 
 FP64 bar: max |phi_gpu - phi_oracle| <= 1e-10 * max |phi_oracle| (the GPU
 transform is a half-spectrum C2R of the symmetrised window; the oracle does
@@ -24,12 +25,13 @@ def ar_surface(shape, seed):
     return O.ar_generate(rng.standard_normal(shape), phi)
 
 
-def run_gpu(ctx, zeta, depth, lb, ub, npts, precision=arma_amd.FP64, want_wn=False):
+def run_gpu(ctx, zeta, depth, lb, ub, npts, precision=arma_amd.FP64, want_wn=False,
+            solver="linear", delta=(TWO_PI,) * 3):
     dt = torch.float64 if precision == arma_amd.FP64 else torch.float32
     z = torch.from_numpy(np.ascontiguousarray(zeta)).to(DEV, dt)
     out = torch.empty((npts[0], npts[1], zeta.shape[1], zeta.shape[2]), dtype=dt, device=DEV)
-    d = ctx.pot_desc(zeta.shape, (TWO_PI,) * 3, depth, lb, ub, npts, precision)
-    wn = ctx.linear_potential(d, z, out, want_wnmax=want_wn)
+    d = ctx.pot_desc(zeta.shape, delta, depth, lb, ub, npts, precision)
+    wn = ctx.linear_potential(d, z, out, want_wnmax=want_wn, solver=solver)
     return out.cpu().numpy().astype(np.float64), wn
 
 
@@ -90,3 +92,60 @@ def test_potential_errors(hip_ctx):
         run_gpu(hip_ctx, zeta, 5.0, (0.0, -6.0), (3.0, 0.0), (4, 2))
     with pytest.raises(ValueError):
         run_gpu(hip_ctx, ar_surface((4, 12, 16), 4), 12.0, (0.0, -6.0), (3.0, 0.0), (4, 2))
+
+
+# ---- high-amplitude solver ----
+
+@pytest.mark.parametrize("shape,npts,lb,ub,delta", [
+    ((6, 16, 16), (6, 4), (0.0, -10.0), (5.0, 0.0), (TWO_PI,) * 3),
+    ((9, 32, 64), (3, 5), (2.0, -8.0), (6.0, 1.0), (TWO_PI, 0.5, 0.5)),
+    ((2, 8, 16), (2, 3), (0.0, -2.0), (1.0, 0.0), (1.0, 0.3, 0.7)),
+    ((12, 64, 32), (12, 16), (0.0, -10.0), (11.0, 0.0), (TWO_PI, 1.0, 0.25)),
+])
+def test_high_amplitude_fp64_matches_oracle(hip_ctx, shape, npts, lb, ub, delta):
+    print(f"high_amplitude fp64 {shape} {npts}", flush=True)
+    zeta = ar_surface(shape, 5)
+    got, wn = run_gpu(hip_ctx, zeta, 12.0, lb, ub, npts, want_wn=True, solver="high_amplitude",
+                      delta=delta)
+    ref, wref = O.high_amplitude_potential(zeta, delta, 12.0, lb, ub, npts, return_wnmax=True)
+    np.testing.assert_allclose(wn, wref, rtol=1e-15)
+    assert np.max(np.abs(got - ref)) <= 1e-10 * np.max(np.abs(ref))
+
+
+def test_high_amplitude_fp32(hip_ctx):
+    zeta = ar_surface((8, 64, 64), 6)
+    npts, lb, ub, delta = (8, 8), (0.0, -10.0), (7.0, 0.0), (TWO_PI, 0.5, 0.5)
+    got, _ = run_gpu(hip_ctx, zeta, 12.0, lb, ub, npts, precision=arma_amd.FP32,
+                     solver="high_amplitude", delta=delta)
+    ref = O.high_amplitude_potential(zeta, delta, 12.0, lb, ub, npts)
+    err = np.abs(got - ref)
+    assert np.max(err) <= 2e-4 * np.max(np.abs(ref))
+    assert np.sqrt(np.mean(err ** 2)) <= 1e-5 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("a,b,c", [(0.3, -0.2, 0.7), (1.5, 2.0, -0.4)])
+def test_high_amplitude_plane(hip_ctx, a, b, c):
+    """Plane zeta = a x + b y + c t: constant multiplier, phi = 4 pi Re(c / D)."""
+    delta = (TWO_PI, 0.5, 0.25)
+    nt, nx, ny = 5, 8, 16
+    t = np.arange(nt)[:, None, None] * delta[0]
+    x = np.arange(nx)[None, :, None] * delta[1]
+    y = np.arange(ny)[None, None, :] * delta[2]
+    zeta = a * x + b * y + c * t + 0.0 * (t + x + y)
+    got, _ = run_gpu(hip_ctx, zeta, 12.0, (0.0, -6.0), (4.0, 0.0), (5, 3),
+                     solver="high_amplitude", delta=delta)
+    s = np.sqrt(1 + a * a + b * b)
+    expect = 4 * np.pi * (c / complex(-1 / s, (a + b) / s - a - b)).real
+    np.testing.assert_allclose(got, expect, rtol=1e-12)
+
+
+def test_high_amplitude_host_pointers_and_errors(hip_ctx):
+    zeta = ar_surface((4, 16, 16), 7)
+    npts = (4, 3)
+    d = hip_ctx.pot_desc(zeta.shape, (TWO_PI,) * 3, 12.0, (0.0, -5.0), (3.0, 0.0), npts)
+    out = np.empty((4, 3, 16, 16), np.float64)
+    hip_ctx.high_amplitude_potential(d, np.ascontiguousarray(zeta), out, on_device=False)
+    ref = O.high_amplitude_potential(zeta, (TWO_PI,) * 3, 12.0, (0.0, -5.0), (3.0, 0.0), npts)
+    assert np.max(np.abs(out - ref)) <= 1e-10 * np.max(np.abs(ref))
+    with pytest.raises(ValueError):
+        run_gpu(hip_ctx, zeta, 5.0, (0.0, -6.0), (3.0, 0.0), (4, 2), solver="high_amplitude")
