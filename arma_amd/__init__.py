@@ -1,5 +1,5 @@
 """MI355X-native ARMA hot path: white noise, 3-D AR surface recurrence and
-the linear FFT velocity potential, as hand-written HIP kernels behind the C
+the linear and high-amplitude FFT velocity potentials, as hand-written HIP kernels behind the C
 ABI of include/arma_hip.h (libarma_hip.so).
 
 The C++ drop-in for the reference's generator and solver registry lives in
@@ -9,6 +9,6 @@ and benchmarks.
 from ._abi import (AR_PLAIN, AR_WAVEFRONT, BATCH_AUTO, BATCH_INTERLEAVED,  # noqa: F401
                    BATCH_SURFACE_MAJOR, FP32, FP64, NOISE_DOUBLE, NOISE_FLOAT,
                    NOISE_PARTITIONED, NOISE_SEQUENTIAL, ArmaHipError, Batch, Context, MtParams,
-                   Server,
+                   Server, Writer,
                    PRNGError, dcmt_create, dcmt_create_many, lib, mt19937_params,
                    read_mt_config, write_mt_config)

@@ -44,7 +44,8 @@ EXPORTS = [
     "arma_hip_server_close_stream", "arma_hip_server_submit", "arma_hip_server_poll",
     "arma_hip_server_release", "arma_hip_server_stats", "arma_hip_server_copy",
     "arma_hip_yule_walker_choi_device", "arma_hip_white_noise_variance_device",
-    "arma_hip_validate_process",
+    "arma_hip_validate_process", "arma_hip_writer_open", "arma_hip_writer_append",
+    "arma_hip_writer_close",
 ]
 
 
@@ -183,6 +184,9 @@ def lib():
         L.arma_hip_white_noise_variance.restype = C.c_double
         L.arma_hip_partition_shape.argtypes = [i3, i3, i3, C.c_int32, i3]
         L.arma_hip_validate_process.argtypes = [dp, i3, C.POINTER(C.c_int32), vp, C.c_int32]
+        L.arma_hip_writer_open.argtypes = [vp, C.c_char_p, C.c_int32, C.c_size_t, C.POINTER(vp)]
+        L.arma_hip_writer_append.argtypes = [vp, vp, C.c_size_t]
+        L.arma_hip_writer_close.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]
         L.arma_hip_partition_shape.restype = None
         L.arma_hip_batch_create.argtypes = [vp, C.POINTER(BatchDesc), C.POINTER(vp)]
         L.arma_hip_batch_destroy.argtypes = [vp]
@@ -346,6 +350,36 @@ def validate_process(phi):
         err.roots = roots
         raise err
     return nbad.value, roots
+
+
+class Writer:
+    """Streams device arrays to a file in network byte order
+    (arma_hip_writer_*): append() returns at once, the bytes go to the file
+    while the context's stream goes on; close() waits and reports
+    (bytes written, seconds from the data being ready to the last write)."""
+
+    def __init__(self, ctx, path, element_size, slab_bytes=0):
+        self.h = C.c_void_p()
+        check(lib().arma_hip_writer_open(ctx.h, os.fsencode(path), element_size, slab_bytes,
+                                         C.byref(self.h)))
+        self._keep = []
+
+    def append(self, data):
+        """data: a device tensor that stays unchanged until close()."""
+        self._keep.append(data)
+        check(lib().arma_hip_writer_append(self.h, C.c_void_p(_ptr(data)),
+                                           data.numel() * data.element_size()))
+
+    def close(self):
+        if not self.h:
+            return None
+        n, t = C.c_uint64(), C.c_double()
+        h, self.h = self.h, C.c_void_p()
+        try:
+            check(lib().arma_hip_writer_close(h, C.byref(n), C.byref(t)))
+        finally:
+            self._keep = []
+        return n.value, t.value
 
 
 def partition_shape(partition, grid, order, parallelism):

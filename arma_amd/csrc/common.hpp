@@ -6,6 +6,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -94,6 +95,13 @@ struct arma_hip_ctx {
 	// batch configurations chosen by the autotuner, by batch signature:
 	// tile width, schedule, workgroups per CU
 	std::map<std::string, std::vector<int>> batch_tuned;
+	// slabs of the streaming writer (pinned host and device staging), kept
+	// for the next writer on this context; writer_cache_busy while one uses them
+	void* writer_host = nullptr;
+	arma_hip::Scratch writer_stage;
+	size_t writer_bytes = 0;
+	bool writer_cache_busy = false;
+	std::mutex writer_mutex;
 };
 
 namespace arma_hip {

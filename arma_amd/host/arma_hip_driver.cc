@@ -315,15 +315,52 @@ Array3D<T> AR_model<T>::do_generate() {
 	const char* kern = std::getenv("ARMA_AR_KERNEL");
 	d.algorithm = (kern && std::string(kern) == "plain") ? ARMA_AR_PLAIN : ARMA_AR_WAVEFRONT;
 	Array3D<T> zeta(shape);
-	Device_buffer buf(zeta.size() * sizeof(T));
-	check(arma_hip_ar_generate(ctx_->get(), &d, buf.p, 1));
+	const size_t bytes = zeta.size() * sizeof(T);
+	std::shared_ptr<void> dev(new Device_buffer(bytes), [](void* b) {
+		delete static_cast<Device_buffer*>(b);
+	});
+	void* dz = static_cast<Device_buffer*>(dev.get())->p;
+	check(arma_hip_ar_generate(ctx_->get(), &d, dz, 1));
 	// post-rescale of Basic_ARMA_model::generate (src/generator/basic_arma_model.cc:102-106)
 	double factor = 0.0;
-	check(arma_hip_rescale(ctx_->get(), buf.p, shape.data(), d.precision, acf_[0], &factor));
+	check(arma_hip_rescale(ctx_->get(), dz, shape.data(), d.precision, acf_[0], &factor));
+	if (writes_in_parallel()) {
+		// src/generator/ar_model_parallel.cc:112-145: zeta.bin is written while
+		// the run goes on.  The writer starts once the surface is final (after
+		// the rescale) and streams it while the GPU computes the velocity
+		// potential; the device copy lives until the writer is done.
+		finish_writes();
+		check(arma_hip_writer_open(ctx_->get(), "zeta.bin", int32_t(sizeof(T)), 0, &writer_));
+		check(arma_hip_writer_append(writer_, dz, bytes));
+		surface_dev_ = dev;
+	}
 	check(arma_hip_synchronize(ctx_->get()));
-	if (hipMemcpy(zeta.data.data(), buf.p, zeta.size() * sizeof(T), hipMemcpyDeviceToHost) != hipSuccess)
+	if (hipMemcpy(zeta.data.data(), dz, bytes, hipMemcpyDeviceToHost) != hipSuccess)
 		throw std::runtime_error("hipMemcpy failed");
 	return zeta;
+}
+
+template <class T>
+AR_model<T>::~AR_model() {
+	if (writer_) (void)arma_hip_writer_close(writer_, nullptr, nullptr);
+}
+
+template <class T>
+bool AR_model<T>::writes_in_parallel() const {
+	return this->oflags_.isset(Output_flags::Binary);
+}
+
+template <class T>
+void AR_model<T>::finish_writes() {
+	if (!writer_) return;
+	arma_hip_writer* w = writer_;
+	writer_ = nullptr;
+	uint64_t nbytes = 0;
+	double secs = 0.0;
+	const int rc = arma_hip_writer_close(w, &nbytes, &secs);
+	surface_dev_.reset();
+	check(rc);
+	std::clog << "Wrote zeta.bin: " << nbytes << " bytes, writer busy " << secs << " s\n";
 }
 
 template <class T>
@@ -486,10 +523,11 @@ template <class T>
 void ARMA_driver<T>::write_all() {
 	// src/arma_driver.cc:29-69,121-130
 	const Output_flags f = model_->oflags();
+	model_->finish_writes();
 	if (!f.isset(Output_flags::Surface)) return;
 	const Grid<3>& g = model_->grid();
 	const int nt = zeta_.shape[0], nx = zeta_.shape[1], ny = zeta_.shape[2];
-	if (f.isset(Output_flags::Binary)) {
+	if (f.isset(Output_flags::Binary) && !model_->writes_in_parallel()) {
 		std::ofstream out("zeta.bin", std::ios::binary);
 		write_big_endian(out, zeta_.data);
 	}

@@ -128,6 +128,11 @@ public:
 	virtual void describe(std::ostream& out) const = 0;
 	const Grid<3>& grid() const { return outgrid_; }
 	Output_flags oflags() const { return oflags_; }
+	// true when the model writes the binary surface itself while the run goes
+	// on (src/generator/basic_model.hh:101-103, src/generator/ar_model.hh:60-62)
+	virtual bool writes_in_parallel() const { return false; }
+	// waits until the surface the model writes is in its file
+	virtual void finish_writes() {}
 
 protected:
 	Grid<3> outgrid_;
@@ -142,6 +147,7 @@ template <class T>
 class AR_model : public Basic_model<T> {
 public:
 	explicit AR_model(std::shared_ptr<Hip_context> ctx) : ctx_(std::move(ctx)) {}
+	~AR_model() override;
 	void read(std::istream& in) override;
 	Array3D<T> generate() override;
 	void prepare() override;
@@ -153,9 +159,14 @@ public:
 	// device-resident surface of the last generate() (Array3D on the host)
 	void determine_coefficients(bool on_device);
 	Array3D<T> do_generate();
+	bool writes_in_parallel() const override;
+	void finish_writes() override;
 
 private:
 	std::shared_ptr<Hip_context> ctx_;
+	// zeta.bin streaming: the writer and the device surface it reads
+	arma_hip_writer* writer_ = nullptr;
+	std::shared_ptr<void> surface_dev_;
 	std::vector<double> acf_;
 	Shape3D acf_shape_{0, 0, 0};
 	Grid<3> acf_grid_;

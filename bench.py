@@ -21,7 +21,9 @@ import ctypes as C
 import json
 import math
 import os
+import shutil
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -43,6 +45,13 @@ WORKLOADS = {
         acf=("propagating", 0.8, (-0.8, 0.0)), shape=(1024, 256, 256), order=7,
         partition=(16, 32, 32), seed=0, depth=12.0, lbound=(0.0, -10.0),
         ubound=(1023.0, 0.0), npoints=(1024, 16), precision="fp64", solver="high_amplitude"),
+    # the AR + potential step with output=binary: zeta.bin (big-endian) is
+    # streamed by the writer while the potential computes; the step ends when
+    # the file is written
+    "ar-potential-zeta-bin-1024x256x256-fp64": dict(
+        acf=("propagating", 0.8, (-0.8, 0.0)), shape=(1024, 256, 256), order=7,
+        partition=(16, 32, 32), seed=0, depth=12.0, lbound=(0.0, -10.0),
+        ubound=(1023.0, 0.0), npoints=(1024, 16), precision="fp64", write_surface=True),
     # precision variant: FP32 storage, compensated accumulation, FP64 setup
     "ar-potential-2048x512x512-fp32": dict(
         acf=("propagating", 0.8, (-0.8, 0.0)), shape=(2048, 512, 512), order=7,
@@ -529,6 +538,9 @@ def main():
     ev = {k: (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
           for k in ("surface", "potential")}
     stage_ms = {"surface": 0.0, "potential": 0.0}
+    writes = {"bytes": 0, "busy_s": 0.0, "steps": 0}
+    out_dir = (tempfile.mkdtemp(prefix="arma_bench_", dir=os.environ.get("ARMA_BENCH_OUT_DIR"))
+               if w.get("write_surface") else None)
 
     def step(timed):
         if timed:
@@ -537,12 +549,25 @@ def main():
         ctx.rescale(zeta, w["shape"], prec, gamma0)
         if timed:
             ev["surface"][1].record()
+        writer = None
+        if out_dir:
+            # zeta.bin streams out while the potential computes
+            # a new file per step: truncating the last one would be timed too
+            writes["n"] = writes.get("n", 0) + 1
+            writer = arma_amd.Writer(ctx, os.path.join(out_dir, f"zeta{writes['n']}.bin"), esize)
+            writer.append(zeta)
         if has_pot:
             if timed:
                 ev["potential"][0].record()
             ctx.linear_potential(pdesc, zeta, phi_out, solver=solver)
             if timed:
                 ev["potential"][1].record()
+        if writer is not None:
+            nbytes, busy = writer.close()
+            if timed:
+                writes["bytes"] += nbytes
+                writes["busy_s"] += busy
+                writes["steps"] += 1
 
     for i in range(args.warmup):
         step(False)
@@ -563,6 +588,8 @@ def main():
     t_total = time.perf_counter() - t_start
     prof = parse_profile(ctx)
     _abi.check(_abi.lib().arma_hip_profile(ctx.h, 0))
+    if out_dir:
+        shutil.rmtree(out_dir, ignore_errors=True)
 
     ms_per_step = 1000.0 * t_total / args.steps
     surf_pts = nt * nx * ny
@@ -635,6 +662,10 @@ def main():
                    "parallelism": "single-gpu"},
         "surface_pts_per_s": surf_rate, "potential_pts_per_s": pot_rate,
         "stage_ms_per_step": {k: v / args.steps for k, v in stage_ms.items()},
+        "zeta_bin_writer": ({"bytes_per_step": writes["bytes"] / writes["steps"],
+                             "writer_busy_ms_per_step": 1e3 * writes["busy_s"] / writes["steps"],
+                             "writer_gb_per_s": writes["bytes"] / writes["busy_s"] / 1e9}
+                            if writes["steps"] else None),
         "roofline": roofline, "kernels": kernels, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -159,6 +159,29 @@ int arma_hip_high_amplitude_potential(arma_hip_ctx* ctx, const arma_pot_desc* de
                                       const void* zeta, void* phi, int on_device,
                                       double* wnmax_out);
 
+/* Streaming writer of device arrays to a file in network byte order: the
+   zeta.bin format of io::Binary_stream (src/io/binary_stream.hh:43-70), and
+   the role of the writer thread of the OpenMP generator
+   (src/generator/ar_model_parallel.cc:112-145), which writes finished
+   t-slabs while the run goes on.  A copy stream byte-swaps (on the GPU) and
+   copies slabs of slab_bytes (0 = 32 MiB) into pinned slots, four in
+   flight, and I/O threads write them at their file offsets, while the
+   context's stream goes on with later work.  element_size is 4 or 8.  Close
+   every writer before destroying its context. */
+typedef struct arma_hip_writer arma_hip_writer;
+int arma_hip_writer_open(arma_hip_ctx* ctx, const char* path, int32_t element_size,
+                         size_t slab_bytes, arma_hip_writer** out);
+/* Appends nbytes at the device pointer data once the work enqueued so far on
+   the context's stream has finished.  data must stay allocated and unchanged
+   until arma_hip_writer_close returns.  Returns the first error of an
+   earlier append, if any. */
+int arma_hip_writer_append(arma_hip_writer* w, const void* data, size_t nbytes);
+/* Waits for every append, closes the file and frees the writer (also on
+   error).  bytes_out and seconds_out (optional) receive the bytes written and
+   the time from the first appended data being ready on the device to the
+   last byte written. */
+int arma_hip_writer_close(arma_hip_writer* w, uint64_t* bytes_out, double* seconds_out);
+
 /* Raw tempered uint32 draws of n_streams streams (draws_per_stream each)
    into out (device), stream-major: the bit-exact check of the generator. */
 int arma_hip_mt_raw(arma_hip_ctx* ctx, const arma_mt_params* mts, const uint32_t* seeds,
