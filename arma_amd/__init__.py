@@ -8,7 +8,8 @@ and benchmarks.
 """
 from ._abi import (AR_PLAIN, AR_WAVEFRONT, BATCH_AUTO, BATCH_INTERLEAVED,  # noqa: F401
                    BATCH_SURFACE_MAJOR, FP32, FP64, NOISE_DOUBLE, NOISE_FLOAT,
-                   NOISE_PARTITIONED, NOISE_SEQUENTIAL, ArmaHipError, Batch, Context, MtParams,
+                   NIT_GRAM_CHARLIER, NIT_SKEW_NORMAL, NOISE_PARTITIONED, NOISE_SEQUENTIAL,
+                   ArmaHipError, nit_desc, Batch, Context, MtParams,
                    Server, Writer,
                    PRNGError, dcmt_create, dcmt_create_many, lib, mt19937_params,
                    read_mt_config, write_mt_config)

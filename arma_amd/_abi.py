@@ -44,7 +44,7 @@ EXPORTS = [
     "arma_hip_server_close_stream", "arma_hip_server_submit", "arma_hip_server_poll",
     "arma_hip_server_release", "arma_hip_server_stats", "arma_hip_server_copy",
     "arma_hip_yule_walker_choi_device", "arma_hip_white_noise_variance_device",
-    "arma_hip_validate_process", "arma_hip_writer_open", "arma_hip_writer_append",
+    "arma_hip_validate_process", "arma_hip_nit_realisation", "arma_hip_writer_open", "arma_hip_writer_append",
     "arma_hip_writer_close",
 ]
 
@@ -78,6 +78,32 @@ class PotDesc(C.Structure):
         ("lbound", C.c_double * 2), ("ubound", C.c_double * 2), ("npoints", C.c_int32 * 2),
         ("precision", C.c_int32),
     ]
+
+
+class NitDesc(C.Structure):
+    """Mirror of arma_nit_desc (transform = nit { ... } of the reference)."""
+    _fields_ = [
+        ("distribution", C.c_int32), ("skewness", C.c_double), ("kurtosis", C.c_double),
+        ("mean", C.c_double), ("stdev", C.c_double), ("alpha", C.c_double),
+        ("interval", C.c_double * 2), ("absolute_error", C.c_double),
+        ("max_iterations", C.c_int32),
+    ]
+
+
+NIT_GRAM_CHARLIER, NIT_SKEW_NORMAL = 0, 1
+
+
+def nit_desc(distribution, skewness=0.0, kurtosis=3.0, mean=0.0, stdev=1.0, alpha=0.0,
+             interval=(0.0, 1.0), absolute_error=1e-6, max_iterations=100):
+    """Defaults are the reference's (src/nonlinear/nit_transform.hh:48-66)."""
+    d = NitDesc()
+    d.distribution = distribution
+    d.skewness, d.kurtosis = skewness, kurtosis
+    d.mean, d.stdev, d.alpha = mean, stdev, alpha
+    d.interval[:] = [float(interval[0]), float(interval[1])]
+    d.absolute_error = absolute_error
+    d.max_iterations = max_iterations
+    return d
 
 
 class BatchConfig(C.Structure):
@@ -184,6 +210,8 @@ def lib():
         L.arma_hip_white_noise_variance.restype = C.c_double
         L.arma_hip_partition_shape.argtypes = [i3, i3, i3, C.c_int32, i3]
         L.arma_hip_validate_process.argtypes = [dp, i3, C.POINTER(C.c_int32), vp, C.c_int32]
+        L.arma_hip_nit_realisation.argtypes = [vp, vp, i3, C.c_int32, C.c_double,
+                                                C.POINTER(NitDesc)]
         L.arma_hip_writer_open.argtypes = [vp, C.c_char_p, C.c_int32, C.c_size_t, C.POINTER(vp)]
         L.arma_hip_writer_append.argtypes = [vp, vp, C.c_size_t]
         L.arma_hip_writer_close.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]
@@ -497,6 +525,12 @@ class Context:
     def high_amplitude_potential(self, desc, zeta, phi, on_device=True, want_wnmax=False):
         return self.linear_potential(desc, zeta, phi, on_device, want_wnmax,
                                      solver="high_amplitude")
+
+    def nit_realisation(self, zeta, shape, precision, gamma0, desc):
+        """transform = nit on the device surface zeta, in place."""
+        check(lib().arma_hip_nit_realisation(self.h, C.c_void_p(_ptr(zeta)),
+                                             (C.c_int32 * 3)(*shape), precision, gamma0,
+                                             C.byref(desc)))
 
     def mt_raw(self, mts, seeds, draws, out):
         marr = mts if isinstance(mts, C.Array) else params_array(list(mts))

@@ -291,6 +291,31 @@ int arma_hip_high_amplitude_potential(arma_hip_ctx* ctx, const arma_pot_desc* de
 	return potential_entry(ctx, desc, zeta, phi, on_device, wnmax_out, 1);
 }
 
+int arma_hip_nit_realisation(arma_hip_ctx* ctx, void* zeta, const int32_t shape[3],
+                             int32_t precision, double gamma0, const arma_nit_desc* desc) {
+	return guarded([&] {
+		use_device(ctx);
+		ARMA_HIP_REQUIRE(zeta && shape && desc, ARMA_HIP_EINVAL, "null argument");
+		for (int k = 0; k < 3; ++k)
+			ARMA_HIP_REQUIRE(shape[k] > 0, ARMA_HIP_EINVAL, "bad surface shape");
+		ARMA_HIP_REQUIRE(precision == ARMA_FP64 || precision == ARMA_FP32, ARMA_HIP_EINVAL,
+		                 "bad precision");
+		ARMA_HIP_REQUIRE(desc->distribution == ARMA_NIT_GRAM_CHARLIER ||
+		                     desc->distribution == ARMA_NIT_SKEW_NORMAL,
+		                 ARMA_HIP_EINVAL, "bad distribution");
+		// src/linalg.cc:246-254
+		ARMA_HIP_REQUIRE(desc->interval[0] < desc->interval[1], ARMA_HIP_EINVAL,
+		                 "bad parameter: bisection.interval");
+		ARMA_HIP_REQUIRE(desc->absolute_error > 0.0, ARMA_HIP_EINVAL,
+		                 "bad parameter: bisection.absolute_error");
+		ARMA_HIP_REQUIRE(desc->max_iterations > 0, ARMA_HIP_EINVAL,
+		                 "bad parameter: bisection.max_iterations");
+		ARMA_HIP_REQUIRE(gamma0 >= 0.0, ARMA_HIP_EINVAL, "negative variance");
+		arma_hip::launch_nit_realisation(ctx, zeta, arma_hip::numel3(shape), precision, gamma0,
+		                                 desc);
+	});
+}
+
 int arma_hip_profile(arma_hip_ctx* ctx, int32_t enable) {
 	return guarded([&] {
 		use_device(ctx);

@@ -231,6 +231,9 @@ void rescale_batch_build(arma_hip_ctx* ctx, RescaleBatch* b, int precision,
 size_t ar_batch_member_bytes(const arma_ar_desc* d, int tile_x);
 void launch_linear_potential(arma_hip_ctx* ctx, const arma_pot_desc* d, const void* zeta,
                              void* phi, double* wnmax_out);
+void launch_nit_realisation(arma_hip_ctx* ctx, void* zeta, size_t n, int precision,
+                            double gamma0, const arma_nit_desc* d);
+
 // solver 0: Linear_solver, 1: High_amplitude_solver
 void launch_potential(arma_hip_ctx* ctx, const arma_pot_desc* d, const void* zeta, void* phi,
                       double* wnmax_out, int solver);

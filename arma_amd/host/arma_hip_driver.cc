@@ -150,6 +150,97 @@ void parameter_map::read(std::istream& in) {
 
 namespace generator {
 
+namespace {
+
+// closed_interval input "[a,b]" (src/apmath/closed_interval.cc:12-28)
+void read_interval(std::istream& in, double (&v)[2], const std::string& name) {
+	if (!expect(in, '[')) throw std::runtime_error("Expecting \"[\" in " + name);
+	in >> v[0];
+	if (!expect(in, ',')) throw std::runtime_error("Expecting \",\" in " + name);
+	in >> v[1];
+	if (!expect(in, ']')) throw std::runtime_error("Expecting \"]\" in " + name);
+	if (!in) throw std::runtime_error("bad interval in " + name);
+}
+
+// Bisection input (src/linalg.cc:238-256)
+void read_bisection(std::istream& in, double (&interval)[2], double& eps, int32_t& niter) {
+	parameter_map p(
+		{
+			{"interval", [&](std::istream& s, const std::string& n) { read_interval(s, interval, n); }},
+			{"absolute_error", [&](std::istream& s, const std::string& n) { read_value(s, eps, n); }},
+			{"max_iterations", [&](std::istream& s, const std::string& n) { read_value(s, niter, n); }},
+		},
+		true);
+	p.read(in);
+	if (!(interval[0] < interval[1])) throw std::runtime_error("bad parameter");
+	if (!(eps > 0)) throw std::runtime_error("bad parameter");
+	if (!(niter > 0)) throw std::runtime_error("bad parameter");
+}
+
+}  // namespace
+
+template <class T>
+void AR_model<T>::read_nit(std::istream& in) {
+	// NIT_transform input (src/nonlinear/nit_transform.cc:206-263) with the
+	// defaults of its constructor (src/nonlinear/nit_transform.hh:48-66)
+	std::memset(&nit_, 0, sizeof nit_);
+	nit_.distribution = ARMA_NIT_GRAM_CHARLIER;
+	nit_.interval[0] = 0.0;
+	nit_.interval[1] = 1.0;
+	nit_.absolute_error = 1e-6;
+	nit_.max_iterations = 100;
+	double acf_interval[2] = {0.0, 1.0};
+	double acf_eps = 1e-6;
+	int32_t acf_niter = 100;
+	long nodes = 100, max_int = 12, max_exp = 10;
+	auto positive = [](long v) {
+		if (!(v > 0)) throw std::runtime_error("bad parameter");
+	};
+	parameter_map p(
+		{
+			{"distribution",
+			 [&](std::istream& s, const std::string& n) {
+				 std::string name;
+				 read_value(s, name, n);
+				 if (name == "gram_charlier") {
+					 nit_.distribution = ARMA_NIT_GRAM_CHARLIER;
+					 parameter_map d(
+						 {
+							 {"skewness", [&](std::istream& q, const std::string& m) { read_value(q, nit_.skewness, m); }},
+							 {"kurtosis", [&](std::istream& q, const std::string& m) { read_value(q, nit_.kurtosis, m); }},
+						 },
+						 true);
+					 d.read(s);
+				 } else if (name == "skew_normal") {
+					 nit_.distribution = ARMA_NIT_SKEW_NORMAL;
+					 parameter_map d(
+						 {
+							 {"mean", [&](std::istream& q, const std::string& m) { read_value(q, nit_.mean, m); }},
+							 {"stdev", [&](std::istream& q, const std::string& m) { read_value(q, nit_.stdev, m); }},
+							 {"alpha", [&](std::istream& q, const std::string& m) { read_value(q, nit_.alpha, m); }},
+						 },
+						 true);
+					 d.read(s);
+				 } else {
+					 throw std::runtime_error("bad distribution");
+				 }
+			 }},
+			{"max_interpolation_order", [&](std::istream& s, const std::string& n) { read_value(s, max_int, n); positive(max_int); }},
+			{"interpolation_nodes", [&](std::istream& s, const std::string& n) { read_value(s, nodes, n); positive(nodes); }},
+			{"max_expansion_order", [&](std::istream& s, const std::string& n) { read_value(s, max_exp, n); positive(max_exp); }},
+			{"cdf_solver",
+			 [&](std::istream& s, const std::string&) {
+				 read_bisection(s, nit_.interval, nit_.absolute_error, nit_.max_iterations);
+			 }},
+			{"acf_solver",
+			 [&](std::istream& s, const std::string&) {
+				 read_bisection(s, acf_interval, acf_eps, acf_niter);
+			 }},
+		},
+		true);
+	p.read(in);
+}
+
 template <class T>
 void AR_model<T>::read(std::istream& in) {
 	std::string transform = "none";
@@ -169,9 +260,16 @@ void AR_model<T>::read(std::istream& in) {
 			{"no_seed", [&](std::istream& s, const std::string& n) { read_value(s, this->noseed_, n); }},
 			{"transform",
 			 [&](std::istream& s, const std::string& n) {
+				 // src/bits/transform_wrapper.hh:25-39
 				 read_value(s, transform, n);
-				 if (transform != "none")
-					 throw std::runtime_error("non-linear transform is not part of the HIP build");
+				 if (transform == "nit") {
+					 read_nit(s);
+					 linear_ = false;
+				 } else if (transform == "none") {
+					 linear_ = true;
+				 } else {
+					 throw std::runtime_error("bad transform");
+				 }
 			 }},
 			{"acf",
 			 [&](std::istream& s, const std::string&) {
@@ -324,6 +422,9 @@ Array3D<T> AR_model<T>::do_generate() {
 	// post-rescale of Basic_ARMA_model::generate (src/generator/basic_arma_model.cc:102-106)
 	double factor = 0.0;
 	check(arma_hip_rescale(ctx_->get(), dz, shape.data(), d.precision, acf_[0], &factor));
+	// src/generator/basic_arma_model.cc:107-111
+	if (!linear_)
+		check(arma_hip_nit_realisation(ctx_->get(), dz, shape.data(), d.precision, acf_[0], &nit_));
 	if (writes_in_parallel()) {
 		// src/generator/ar_model_parallel.cc:112-145: zeta.bin is written while
 		// the run goes on.  The writer starts once the surface is final (after

@@ -163,6 +163,10 @@ public:
 	void finish_writes() override;
 
 private:
+	void read_nit(std::istream& in);
+	// transform = nit (src/generator/basic_arma_model.cc:22-25)
+	bool linear_ = true;
+	arma_nit_desc nit_{};
 	std::shared_ptr<Hip_context> ctx_;
 	// zeta.bin streaming: the writer and the device surface it reads
 	arma_hip_writer* writer_ = nullptr;

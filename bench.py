@@ -52,6 +52,13 @@ WORKLOADS = {
         acf=("propagating", 0.8, (-0.8, 0.0)), shape=(1024, 256, 256), order=7,
         partition=(16, 32, 32), seed=0, depth=12.0, lbound=(0.0, -10.0),
         ubound=(1023.0, 0.0), npoints=(1024, 16), precision="fp64", write_surface=True),
+    # surface with transform = nit: Gram-Charlier target with the parameters of
+    # the reference's input/nit-standing-gramcharlier (skewness 3.25, kurtosis
+    # 2.4, cdf_solver interval [-5,5], absolute error 1e-6, 100 iterations)
+    "ar-nit-1024x256x256-fp64": dict(
+        acf=("propagating", 0.8, (-0.8, 0.0)), shape=(1024, 256, 256), order=7,
+        partition=(16, 32, 32), seed=0, depth=None, precision="fp64",
+        nit=dict(distribution=0, skewness=3.25, kurtosis=2.4, interval=(-5.0, 5.0))),
     # precision variant: FP32 storage, compensated accumulation, FP64 setup
     "ar-potential-2048x512x512-fp32": dict(
         acf=("propagating", 0.8, (-0.8, 0.0)), shape=(2048, 512, 512), order=7,
@@ -534,6 +541,10 @@ def main():
                              w["ubound"], w["npoints"], prec)
         phi_out = torch.empty((w["npoints"][0], nz, nx, ny), dtype=tdt, device="cuda:0")
     gamma0 = float(acf[0, 0, 0])
+    nit = None
+    if w.get("nit"):
+        spec = dict(w["nit"])
+        nit = arma_amd.nit_desc(spec.pop("distribution"), **spec)
 
     ev = {k: (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
           for k in ("surface", "potential")}
@@ -547,6 +558,8 @@ def main():
             ev["surface"][0].record()
         ctx.ar_generate(desc, zeta)
         ctx.rescale(zeta, w["shape"], prec, gamma0)
+        if nit is not None:
+            ctx.nit_realisation(zeta, w["shape"], prec, gamma0, nit)
         if timed:
             ev["surface"][1].record()
         writer = None
@@ -658,6 +671,7 @@ def main():
                    "noise_streams": nstreams, "ar_kernel": args.ar,
                    "potential": list(w["npoints"]) + [nx, ny] if has_pot else None,
                    "solver": solver if has_pot else None,
+                   "transform": "nit gram_charlier" if nit is not None else "none",
                    "setup": "FP64 Yule-Walker (Choi) and white-noise variance on the GPU",
                    "parallelism": "single-gpu"},
         "surface_pts_per_s": surf_rate, "potential_pts_per_s": pot_rate,

@@ -112,6 +112,19 @@ typedef struct arma_pot_desc {
 	int32_t precision;           /* enum arma_precision */
 } arma_pot_desc;
 
+/* Target law of the non-linear inertialess transform (the "distribution"
+   parameter of transform = nit, src/nonlinear/nit_transform.cc:206-223). */
+enum arma_nit_distribution { ARMA_NIT_GRAM_CHARLIER = 0, ARMA_NIT_SKEW_NORMAL = 1 };
+
+typedef struct arma_nit_desc {
+	int32_t distribution;        /* enum arma_nit_distribution */
+	double skewness, kurtosis;   /* gram_charlier { skewness= kurtosis= } */
+	double mean, stdev, alpha;   /* skew_normal { mean= stdev= alpha= } */
+	double interval[2];          /* cdf_solver.interval [a, b], a < b */
+	double absolute_error;       /* cdf_solver.absolute_error (> 0; default 1e-6) */
+	int32_t max_iterations;      /* cdf_solver.max_iterations (> 0; default 100) */
+} arma_nit_desc;
+
 typedef struct arma_hip_ctx arma_hip_ctx;
 
 int arma_hip_init(int device, arma_hip_ctx** ctx);
@@ -158,6 +171,19 @@ int arma_hip_linear_potential(arma_hip_ctx* ctx, const arma_pot_desc* desc, cons
 int arma_hip_high_amplitude_potential(arma_hip_ctx* ctx, const arma_pot_desc* desc,
                                       const void* zeta, void* phi, int on_device,
                                       double* wnmax_out);
+
+/* NIT_transform<T>::transform_realisation (src/nonlinear/nit_transform.cc:
+   127-164, called at src/generator/basic_arma_model.cc:107-111 when
+   transform = nit): every point x of zeta (device, nt*nx*ny elements of
+   precision) becomes the root in interval of F(c) = Phi(x / sqrt(gamma0)),
+   F the target CDF, by the reference's bisection (src/linalg.hh:199-213).
+   Returns ARMA_HIP_EINVAL for an empty interval, a non-positive absolute
+   error or iteration count, or gamma0 < 0.  The reference's transform_ACF
+   works on a copy of the ACF that is then dropped
+   (src/generator/basic_arma_model.cc:74-79) and changes no output; it has no
+   counterpart here. */
+int arma_hip_nit_realisation(arma_hip_ctx* ctx, void* zeta, const int32_t shape[3],
+                             int32_t precision, double gamma0, const arma_nit_desc* desc);
 
 /* Streaming writer of device arrays to a file in network byte order: the
    zeta.bin format of io::Binary_stream (src/io/binary_stream.hh:43-70), and

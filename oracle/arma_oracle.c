@@ -35,6 +35,20 @@
  *                       rotate index map.
  *   orc_linear_potential pinned: single-Fourier-mode analytic solution and
  *                       numpy restatement on 16x16 grids.
+ *   orc_gauss_legendre_half, orc_owen_t
+ *                       pinned: numpy's leggauss(20) nodes and weights;
+ *                       T(0, a) = atan(a) / (2 pi), T(h, 1) =
+ *                       Phi(h) (1 - Phi(h)) / 2, and adaptive quadrature.
+ *   orc_gram_charlier_cdf, orc_skew_normal_cdf
+ *                       pinned: Gram-Charlier with skewness 0 and kurtosis 3
+ *                       is Phi; its derivative is the Gram-Charlier A density
+ *                       phi (1 + s He3 / 6 + (k - 3) He4 / 24); skew normal
+ *                       with alpha 0 is the normal CDF, its derivative the
+ *                       skew normal density 2 phi(z) Phi(alpha z) / sigma.
+ *   orc_nit_realisation pinned: brute-force Python restatement of the
+ *                       bisection (src/linalg.hh:199-213), and the identity
+ *                       transforms (Gram-Charlier (0, 3) maps x to x/stdev,
+ *                       skew normal alpha 0 to mean + sigma x / stdev).
  *   orc_derivative_xy   pinned: exact on quadratics in x and y, raw slice for
  *                       extents below 3.
  *   orc_high_amplitude_potential
@@ -1139,4 +1153,102 @@ int orc_high_amplitude_potential(const double* zeta, const int shape[3], const d
                                  double depth, const double lbound[2], const double ubound[2],
                                  const int npoints[2], double* out, double* wnmax_out) {
 	return potential(zeta, shape, delta, depth, lbound, ubound, npoints, out, wnmax_out, 1);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Non-linear inertialess transform of the realisation                      */
+/* (NIT_transform::transform_realisation, src/nonlinear/nit_transform.cc:    */
+/* 127-164, transform_data src/nonlinear/transforms.hh:58-75).               */
+/* ------------------------------------------------------------------------ */
+
+/* Positive nodes and weights of the 2m-point Gauss-Legendre rule on [-1, 1]
+   (roots of the Legendre polynomial P_2m by Newton's method from the
+   Chebyshev-like initial guesses; w = 2 / ((1 - x^2) P'(x)^2)). */
+void orc_gauss_legendre_half(int m, double* x, double* w) {
+	const int n = 2 * m;
+	for (int i = 0; i < m; ++i) {
+		/* the i-th root from the top, ordered ascending at the end */
+		double z = cos(M_PI * (i + 0.75) / (n + 0.5)), dp = 0.0;
+		for (int it = 0; it < 100; ++it) {
+			double p0 = 1.0, p1 = z;
+			for (int k = 2; k <= n; ++k) {
+				const double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+				p0 = p1;
+				p1 = p2;
+			}
+			dp = n * (z * p1 - p0) / (z * z - 1.0);
+			const double dz = p1 / dp;
+			z -= dz;
+			if (fabs(dz) < 1e-15) break;  /* converged to within a few ulp */
+		}
+		x[m - 1 - i] = z;
+		w[m - 1 - i] = 2.0 / ((1.0 - z * z) * dp * dp);
+	}
+}
+
+/* Owen's T(h, a) by the symmetric 20-point Gauss-Legendre rule
+   (src/apmath/owen_t.cc:38-55). */
+double orc_owen_t(double h, double alpha) {
+	static double x[10], w[10];
+	static int have = 0;
+	if (!have) {
+		orc_gauss_legendre_half(10, x, w);
+		have = 1;
+	}
+	const double a2 = alpha * alpha, h2 = -0.5 * h * h;
+	double r = 0.0;
+	for (int i = 0; i < 10; ++i) {
+		const double term = 1.0 + a2 * x[i] * x[i];
+		r += w[i] * exp(h2 * term) / term;
+	}
+	return r * alpha / (2.0 * M_PI);
+}
+
+/* Gram-Charlier CDF (src/stats/gram_charlier.hh:40-45), with the
+   reference's 13-digit constants sqrt(2 pi) = 2.506628274631 and
+   sqrt(2) = 1.4142135623731 (src/physical_constants.hh:21-24). */
+double orc_gram_charlier_cdf(double x, double skewness, double kurtosis) {
+	const double sqrt2pi = 2.506628274631, sqrt2 = 1.4142135623731;
+	return exp(-0.5 * x * x) *
+	           (kurtosis * (3.0 * x - x * x * x) + skewness * (4.0 - 4.0 * x * x) +
+	            3.0 * x * x * x - 9.0 * x) /
+	           (24.0 * sqrt2pi) +
+	       0.5 * erf(x / sqrt2) + 0.5;
+}
+
+/* Normal CDF with mean mu and deviation sigma (gsl_cdf_gaussian_P). */
+static double normal_cdf(double x, double mu, double sigma) {
+	return 0.5 * erfc(-((x - mu) / sigma) / 1.4142135623730951);
+}
+
+/* Skew normal CDF (src/stats/skew_normal.hh:38-42). */
+double orc_skew_normal_cdf(double x, double mean, double sigma, double alpha) {
+	return normal_cdf(x, mean, sigma) - 2.0 * orc_owen_t((x - mean) / sigma, alpha);
+}
+
+static double target_cdf(int dist, const double p[5], double x) {
+	return dist == 0 ? orc_gram_charlier_cdf(x, p[0], p[1])
+	                 : orc_skew_normal_cdf(x, p[2], p[3], p[4]);
+}
+
+/* Every point x -> the root in [a, b] of F(c) = Phi(x / stdev) by the
+   reference's bisection (src/linalg.hh:199-213): at most max_iter steps,
+   stopping once b - a <= eps or |F(c) - Phi| <= eps; the midpoint of the
+   last step is the result.  dist 0: Gram-Charlier (p[0] skewness, p[1]
+   kurtosis); 1: skew normal (p[2] mean, p[3] stdev, p[4] alpha). */
+void orc_nit_realisation(double* data, size_t n, double stdev, int dist, const double p[5],
+                         double a0, double b0, double eps, int max_iter) {
+	for (size_t q = 0; q < n; ++q) {
+		const double y = normal_cdf(data[q], 0.0, stdev);
+		double a = a0, b = b0, c, fc;
+		int i = 0;
+		do {
+			c = 0.5 * (a + b);
+			fc = target_cdf(dist, p, c) - y;
+			if ((target_cdf(dist, p, a) - y) * fc < 0.0) b = c;
+			if ((target_cdf(dist, p, b) - y) * fc < 0.0) a = c;
+			++i;
+		} while (i < max_iter && (b - a) > eps && fabs(fc) > eps);
+		data[q] = c;
+	}
 }

@@ -117,6 +117,15 @@ int orc_high_amplitude_potential(const double* zeta, const int shape[3], const d
                                  double depth, const double lbound[2], const double ubound[2],
                                  const int npoints[2], double* out, double* wnmax_out);
 
+/* ---- non-linear inertialess transform of the realisation ---- */
+
+void orc_gauss_legendre_half(int m, double* x, double* w);
+double orc_owen_t(double h, double alpha);
+double orc_gram_charlier_cdf(double x, double skewness, double kurtosis);
+double orc_skew_normal_cdf(double x, double mean, double sigma, double alpha);
+void orc_nit_realisation(double* data, size_t n, double stdev, int dist, const double p[5],
+                         double a0, double b0, double eps, int max_iter);
+
 #ifdef __cplusplus
 }
 #endif

@@ -272,3 +272,54 @@ def linear_potential(zeta, delta, depth, lbound, ubound, npoints, return_wnmax=F
 def high_amplitude_potential(zeta, delta, depth, lbound, ubound, npoints, return_wnmax=False):
     return linear_potential(zeta, delta, depth, lbound, ubound, npoints, return_wnmax,
                             solver="high_amplitude")
+
+
+# ---- non-linear inertialess transform of the realisation ----
+
+GRAM_CHARLIER, SKEW_NORMAL = 0, 1
+
+
+def _nit_lib():
+    L = lib()
+    if not getattr(L, "_nit_set", False):
+        dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+        L.orc_gauss_legendre_half.argtypes = [C.c_int, dp, dp]
+        L.orc_owen_t.argtypes = [C.c_double, C.c_double]
+        L.orc_owen_t.restype = C.c_double
+        L.orc_gram_charlier_cdf.argtypes = [C.c_double, C.c_double, C.c_double]
+        L.orc_gram_charlier_cdf.restype = C.c_double
+        L.orc_skew_normal_cdf.argtypes = [C.c_double] * 4
+        L.orc_skew_normal_cdf.restype = C.c_double
+        L.orc_nit_realisation.argtypes = [dp, C.c_size_t, C.c_double, C.c_int, C.c_double * 5,
+                                          C.c_double, C.c_double, C.c_double, C.c_int]
+        L._nit_set = True
+    return L
+
+
+def gauss_legendre_half(m):
+    x = np.empty(m)
+    w = np.empty(m)
+    _nit_lib().orc_gauss_legendre_half(m, x, w)
+    return x, w
+
+
+def owen_t(h, a):
+    return _nit_lib().orc_owen_t(h, a)
+
+
+def gram_charlier_cdf(x, skewness, kurtosis):
+    return _nit_lib().orc_gram_charlier_cdf(x, skewness, kurtosis)
+
+
+def skew_normal_cdf(x, mean, sigma, alpha):
+    return _nit_lib().orc_skew_normal_cdf(x, mean, sigma, alpha)
+
+
+def nit_realisation(data, stdev, dist, skewness=0.0, kurtosis=3.0, mean=0.0, sigma=1.0,
+                    alpha=0.0, interval=(0.0, 1.0), eps=1e-6, max_iter=100):
+    """transform_realisation of the reference on a copy of data (FP64)."""
+    out = np.ascontiguousarray(data, np.float64).copy()
+    p = (C.c_double * 5)(skewness, kurtosis, mean, sigma, alpha)
+    _nit_lib().orc_nit_realisation(out.reshape(-1), out.size, stdev, dist, p, interval[0],
+                                   interval[1], eps, max_iter)
+    return out
