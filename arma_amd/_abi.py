@@ -44,7 +44,7 @@ EXPORTS = [
     "arma_hip_server_close_stream", "arma_hip_server_submit", "arma_hip_server_poll",
     "arma_hip_server_release", "arma_hip_server_stats", "arma_hip_server_copy",
     "arma_hip_yule_walker_choi_device", "arma_hip_white_noise_variance_device",
-    "arma_hip_validate_process", "arma_hip_nit_realisation", "arma_hip_writer_open", "arma_hip_writer_append",
+    "arma_hip_validate_process", "arma_hip_nit_realisation", "arma_hip_acf_generate", "arma_hip_writer_open", "arma_hip_writer_append",
     "arma_hip_writer_close",
 ]
 
@@ -78,6 +78,37 @@ class PotDesc(C.Structure):
         ("lbound", C.c_double * 2), ("ubound", C.c_double * 2), ("npoints", C.c_int32 * 2),
         ("precision", C.c_int32),
     ]
+
+
+class AcfGenDesc(C.Structure):
+    """Mirror of arma_acf_gen_desc (the ACF_generator input block)."""
+    _fields_ = [
+        ("func", C.c_int32), ("amplitude", C.c_double), ("velocity", C.c_double),
+        ("beta", C.c_double * 2), ("alpha", C.c_double * 3), ("nwaves", C.c_double * 3),
+        ("shape", C.c_int32 * 3),
+    ]
+
+
+WAVE_SIN, WAVE_COS, WAVE_STOKES, WAVE_STANDING = 0, 1, 2, 3
+
+
+def acf_generate(func=WAVE_COS, amplitude=1.0, velocity=1.0, beta=(0.8, 0.0),
+                 alpha=(0.06, 0.06, 0.06), nwaves=(1.5, 1.5, 1.5), shape=(16, 16, 16)):
+    """ACF_generator (src/generator/acf_generator.cc:130-215): (acf, delta).
+    Defaults are the reference's (src/generator/acf_generator.hh:25-34)."""
+    d = AcfGenDesc()
+    d.func, d.amplitude, d.velocity = func, amplitude, velocity
+    d.beta[:] = list(beta)
+    d.alpha[:] = list(alpha)
+    d.nwaves[:] = list(nwaves)
+    d.shape[:] = [int(v) for v in shape]
+    cap = 1 << 21
+    out = np.empty(cap, np.float64)
+    so = (C.c_int32 * 3)()
+    de = (C.c_double * 3)()
+    check(lib().arma_hip_acf_generate(C.byref(d), out, cap, so, de))
+    s = tuple(so)
+    return out[:int(np.prod(s))].reshape(s).copy(), tuple(de)
 
 
 class NitDesc(C.Structure):
@@ -210,6 +241,8 @@ def lib():
         L.arma_hip_white_noise_variance.restype = C.c_double
         L.arma_hip_partition_shape.argtypes = [i3, i3, i3, C.c_int32, i3]
         L.arma_hip_validate_process.argtypes = [dp, i3, C.POINTER(C.c_int32), vp, C.c_int32]
+        L.arma_hip_acf_generate.argtypes = [C.POINTER(AcfGenDesc), dp, C.c_size_t,
+                                            C.POINTER(C.c_int32), C.POINTER(C.c_double)]
         L.arma_hip_nit_realisation.argtypes = [vp, vp, i3, C.c_int32, C.c_double,
                                                 C.POINTER(NitDesc)]
         L.arma_hip_writer_open.argtypes = [vp, C.c_char_p, C.c_int32, C.c_size_t, C.POINTER(vp)]

@@ -13,6 +13,7 @@
 //   partition shape ...... src/generator/ar_model.cc:23-50
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstring>
 #include <limits>
 #include <map>
@@ -454,6 +455,97 @@ void arma_hip::chop_shape(const std::vector<double>& phi, const int shape[3], do
 	chop(phi, shape, eps, out);
 }
 
+namespace {
+
+// gsl_stats_variance: running mean and running mean of squared deviations in
+// long double, times n/(n-1).
+double gsl_style_variance(const std::vector<double>& v) {
+	long double mean = 0.0L;
+	for (size_t i = 0; i < v.size(); ++i) mean += (v[i] - mean) / (long double)(i + 1);
+	const double m = double(mean);
+	long double var = 0.0L;
+	for (size_t i = 0; i < v.size(); ++i) {
+		const long double d = v[i] - m;
+		var += (d * d - var) / (long double)(i + 1);
+	}
+	return double(var) * (double(v.size()) / double(v.size() - 1));
+}
+
+double wave_profile(int func, double amp, double kx, double ky, double vel, double x, double y,
+                    double t) {
+	const double two_pi = 2.0 * M_PI;  // T(2) * pi (src/physical_constants.hh:15)
+	const double theta = two_pi * (kx * x + ky * y - vel * t);
+	switch (func) {
+	case ARMA_WAVE_SIN: return amp * std::sin(theta + 0.0);
+	case ARMA_WAVE_COS: return amp * std::sin(theta + (0.0 + 0.5 * M_PI));
+	case ARMA_WAVE_STOKES: {
+		const double st = two_pi * amp * std::sqrt(kx * kx + ky * ky);
+		return amp * (std::cos(theta + 0.0) + 0.5 * st * std::cos(2.0 * (theta + 0.0)) +
+		              (3.0 / 8.0) * st * st * std::cos(3.0 * (theta + 0.0)));
+	}
+	default:
+		return amp * std::cos(two_pi * (kx * x + ky * y) + 0.0) * std::cos(two_pi * (vel * t));
+	}
+}
+
+struct Wave {
+	int s[3];
+	std::vector<double> v;
+};
+
+// The plain wave on Domain(-r, r, ws + 1) (src/generator/acf_generator.hh:156-167).
+Wave make_wave(const arma_acf_gen_desc& d, const int ws[3]) {
+	Wave w;
+	double step[3];
+	for (int q = 0; q < 3; ++q) {
+		w.s[q] = ws[q] + 1;
+		step[q] = (d.nwaves[q] - (-d.nwaves[q])) / ws[q];
+	}
+	w.v.resize(size_t(w.s[0]) * w.s[1] * w.s[2]);
+	size_t q = 0;
+	for (int i = 0; i < w.s[0]; ++i)
+		for (int j = 0; j < w.s[1]; ++j)
+			for (int k = 0; k < w.s[2]; ++k)
+				w.v[q++] = wave_profile(d.func, d.amplitude, d.beta[0], d.beta[1], d.velocity,
+				                        -d.nwaves[1] + step[1] * j, -d.nwaves[2] + step[2] * k,
+				                        -d.nwaves[0] + step[0] * i);
+	return w;
+}
+
+// In-place DFT of every line along axis ax of a (s0, s1, s2) complex array,
+// e^{sign 2 pi i jk/n} from a table of the n roots of unity.
+void dft_axis(std::vector<std::complex<double>>& a, const int s[3], int ax, int sign) {
+	const int n = s[ax];
+	std::vector<std::complex<double>> root(n), line(n), res(n);
+	for (int m = 0; m < n; ++m) {
+		const double ang = 2.0 * M_PI * double(m) / double(n);
+		root[m] = std::complex<double>(std::cos(ang), sign * std::sin(ang));
+	}
+	const size_t stride = ax == 0 ? size_t(s[1]) * s[2] : ax == 1 ? size_t(s[2]) : 1;
+	const int o1 = ax == 0 ? s[1] : s[0], o2 = ax == 2 ? s[1] : s[2];
+	for (int p = 0; p < o1; ++p)
+		for (int r = 0; r < o2; ++r) {
+			size_t base;
+			if (ax == 0) base = size_t(p) * s[2] + r;
+			else if (ax == 1) base = size_t(p) * s[1] * s[2] + r;
+			else base = (size_t(p) * s[1] + r) * s[2];
+			for (int m = 0; m < n; ++m) line[m] = a[base + m * stride];
+			for (int k = 0; k < n; ++k) {
+				std::complex<double> acc(0.0, 0.0);
+				int idx = 0;
+				for (int m = 0; m < n; ++m) {
+					acc += line[m] * root[idx];
+					idx += k;
+					if (idx >= n) idx -= n;
+				}
+				res[k] = acc;
+			}
+			for (int m = 0; m < n; ++m) a[base + m * stride] = res[m];
+		}
+}
+
+}  // namespace
+
 extern "C" {
 
 int arma_hip_acf(int32_t func, const double delta[3], const int32_t shape[3], double amplitude,
@@ -474,6 +566,76 @@ int arma_hip_acf(int32_t func, const double delta[3], const int32_t shape[3], do
 				out[(size_t(i) * shape[1] + j) * shape[2] + k] = amplitude * decay * wave;
 			}
 	return ARMA_HIP_OK;
+}
+
+
+int arma_hip_acf_generate(const arma_acf_gen_desc* desc, double* out, size_t cap,
+                          int32_t shape_out[3], double delta_out[3]) {
+	if (!desc || !out || !shape_out || !delta_out) return ARMA_HIP_EINVAL;
+	if (desc->func < ARMA_WAVE_SIN || desc->func > ARMA_WAVE_STANDING) return ARMA_HIP_EINVAL;
+	try {
+		const arma_acf_gen_desc& d = *desc;
+		int ws[3] = {2, 2, 2};
+		Wave w;
+		if (d.shape[0] > 0 && d.shape[1] > 0 && d.shape[2] > 0) {
+			for (int q = 0; q < 3; ++q) ws[q] = 2 * d.shape[q];
+			w = make_wave(d, ws);
+		} else {
+			// generate_optimal_wavy_surface (src/generator/acf_generator.cc:158-193)
+			double var0 = -1.0, var = -1.0;
+			while (ws[0] < 64 && ws[1] < 64 && ws[2] < 64) {
+				w = make_wave(d, ws);
+				var0 = var;
+				var = gsl_style_variance(w.v);
+				if (!(var0 < 0.0) && std::fabs(var - var0) < 1e-3) break;
+				for (int q = 0; q < 3; ++q) ws[q] *= 2;
+			}
+		}
+		// add_exponential_decay (src/generator/acf_generator.cc:196-215)
+		double step[3];
+		for (int q = 0; q < 3; ++q) step[q] = (d.nwaves[q] - (-d.nwaves[q])) / (w.s[q] - 1);
+		std::vector<double> we(w.v.size());
+		size_t q = 0;
+		for (int i = 0; i < w.s[0]; ++i)
+			for (int j = 0; j < w.s[1]; ++j)
+				for (int k = 0; k < w.s[2]; ++k, ++q) {
+					const double t = -d.nwaves[0] + step[0] * i, x = -d.nwaves[1] + step[1] * j,
+					             y = -d.nwaves[2] + step[2] * k;
+					we[q] = w.v[q] * std::exp(-(std::fabs(t * d.alpha[0]) +
+					                            std::fabs(x * d.alpha[1]) +
+					                            std::fabs(y * d.alpha[2])));
+				}
+		const double f = std::sqrt(gsl_style_variance(w.v) / gsl_style_variance(we));
+		for (double& v : we) v *= f;
+		// auto_covariance (src/arma.cc:81-100)
+		std::vector<std::complex<double>> a(we.begin(), we.end());
+		for (int ax = 0; ax < 3; ++ax) dft_axis(a, w.s, ax, -1);
+		for (auto& c : a) {
+			const double m = std::abs(c);
+			c = std::complex<double>(m * m, 0.0);
+		}
+		for (int ax = 0; ax < 3; ++ax) dft_axis(a, w.s, ax, +1);
+		const double n = double(a.size());
+		int h[3];
+		for (int r = 0; r < 3; ++r) {
+			h[r] = w.s[r] / 2;
+			shape_out[r] = h[r];
+			delta_out[r] = h[r] > 1 ? d.nwaves[r] / double(h[r] - 1) : 0.0;
+		}
+		if (size_t(h[0]) * h[1] * h[2] > cap) {
+			arma_hip::set_last_error("ACF output buffer too small");
+			return ARMA_HIP_EINVAL;
+		}
+		for (int i = 0; i < h[0]; ++i)
+			for (int j = 0; j < h[1]; ++j)
+				for (int k = 0; k < h[2]; ++k)
+					out[(size_t(i) * h[1] + j) * h[2] + k] =
+						a[(size_t(i) * w.s[1] + j) * w.s[2] + k].real() / n / n;
+		return ARMA_HIP_OK;
+	} catch (const std::exception& e) {
+		arma_hip::set_last_error(e.what());
+		return ARMA_HIP_ERUNTIME;
+	}
 }
 
 int arma_hip_yule_walker_choi(const double* acf, int32_t n, int32_t max_order, double var_eps,

@@ -273,11 +273,16 @@ void AR_model<T>::read(std::istream& in) {
 			 }},
 			{"acf",
 			 [&](std::istream& s, const std::string&) {
-				 // closed-form ACF block (src/bits/acf_wrapper.cc:41-70)
+				 // with grid = ...: the closed-form ACF (src/bits/acf_wrapper.cc:41-70);
+				 // without: ACF_generator (src/generator/acf_generator.cc:216-233),
+				 // the form the reference's AR model parses at
+				 // src/generator/basic_arma_model.cc:26,57
 				 std::string func;
 				 double amplitude = 1.0, velocity = 1.0;
 				 std::array<double, 3> alpha{0.06, 0.06, 0.06};
 				 std::array<double, 2> beta{0.8, 0.0};
+				 std::array<double, 3> nwaves{1.5, 1.5, 1.5};
+				 std::array<int, 3> gshape{16, 16, 16};
 				 parameter_map p(
 					 {
 						 {"grid", [&](std::istream& q, const std::string& n) { read_grid(q, acf_grid_, n); }},
@@ -286,9 +291,42 @@ void AR_model<T>::read(std::istream& in) {
 						 {"velocity", [&](std::istream& q, const std::string& n) { read_value(q, velocity, n); }},
 						 {"alpha", [&](std::istream& q, const std::string& n) { read_vec(q, alpha, n); }},
 						 {"beta", [&](std::istream& q, const std::string& n) { read_vec(q, beta, n); }},
+						 {"nwaves", [&](std::istream& q, const std::string& n) { read_vec(q, nwaves, n); }},
+						 {"shape", [&](std::istream& q, const std::string& n) { read_vec(q, gshape, n); }},
 					 },
 					 true);
 				 p.read(s);
+				 if (acf_grid_.npoints[0] <= 0) {
+					 arma_acf_gen_desc g;
+					 std::memset(&g, 0, sizeof g);
+					 // src/generator/plain_wave_profile.cc:7-24
+					 if (func.empty() || func == "cos") g.func = ARMA_WAVE_COS;
+					 else if (func == "sin") g.func = ARMA_WAVE_SIN;
+					 else if (func == "stokes") g.func = ARMA_WAVE_STOKES;
+					 else if (func == "standing_wave") g.func = ARMA_WAVE_STANDING;
+					 else throw std::runtime_error("bad function");
+					 g.amplitude = amplitude;
+					 g.velocity = velocity;
+					 for (int k = 0; k < 3; ++k) {
+						 g.alpha[k] = alpha[k];
+						 g.nwaves[k] = nwaves[k];
+						 g.shape[k] = gshape[k];
+					 }
+					 g.beta[0] = beta[0];
+					 g.beta[1] = beta[1];
+					 std::vector<double> buf(size_t(1) << 21);
+					 int32_t so[3];
+					 double de[3];
+					 check(arma_hip_acf_generate(&g, buf.data(), buf.size(), so, de));
+					 for (int k = 0; k < 3; ++k) {
+						 acf_shape_[k] = so[k];
+						 acf_grid_.npoints[k] = so[k];
+						 acf_grid_.length[k] = nwaves[k];
+					 }
+					 acf_.assign(buf.begin(), buf.begin() + size_t(so[0]) * so[1] * so[2]);
+					 have_acf = true;
+					 return;
+				 }
 				 int f;
 				 if (func == "standing_wave") f = ARMA_ACF_STANDING_WAVE;
 				 else if (func == "propagating_wave") f = ARMA_ACF_PROPAGATING_WAVE;

@@ -241,6 +241,35 @@ enum arma_acf_function {
 	ARMA_ACF_PROPAGATING_WAVE = 1   /* propagating_wave_ACF, src/acf.hh:39-58 */
 };
 
+/* Plain wave profile of the ACF generator (src/generator/plain_wave.hh,
+   input names "sin", "cos", "stokes", "standing_wave"). */
+enum arma_wave_profile {
+	ARMA_WAVE_SIN = 0, ARMA_WAVE_COS = 1, ARMA_WAVE_STOKES = 2, ARMA_WAVE_STANDING = 3
+};
+
+typedef struct arma_acf_gen_desc {
+	int32_t func;            /* enum arma_wave_profile (default cos) */
+	double amplitude;        /* default 1 */
+	double velocity;         /* default 1 */
+	double beta[2];          /* wave numbers (default (0.8, 0)) */
+	double alpha[3];         /* decay (default 0.06 each) */
+	double nwaves[3];        /* half extent of the wave's domain (default 1.5 each) */
+	int32_t shape[3];        /* ACF shape; any zero: chosen by doubling (default 16 each) */
+} arma_acf_gen_desc;
+
+/* ACF_generator<T>::generate (src/generator/acf_generator.cc:130-215), the
+   ACF the reference's AR model uses when its acf block has no grid
+   (src/generator/basic_arma_model.cc:57): the auto-covariance
+   real(IFFT(|FFT(w)|^2))/n^2 (src/arma.cc:81-100) of the plain wave w on the
+   domain [-nwaves, nwaves] with 2*shape+1 points, multiplied by
+   exp(-sum |alpha (t,x,y)|) and rescaled to the wave's variance, cut to its
+   first half in every dimension.  out (capacity cap doubles) receives the
+   ACF, shape_out its shape and delta_out its grid spacing
+   nwaves/(shape_out-1) (Grid{shape, nwaves}).  Returns ARMA_HIP_EINVAL when
+   cap is too small.  Runs on the host: the sizes are at most 129^3. */
+int arma_hip_acf_generate(const arma_acf_gen_desc* desc, double* out, size_t cap,
+                          int32_t shape_out[3], double delta_out[3]);
+
 /* Closed-form ACF on a grid of shape points with spacing delta
    (src/acf.hh:18-62 via src/bits/acf_wrapper.cc:41-70). */
 int arma_hip_acf(int32_t func, const double delta[3], const int32_t shape[3], double amplitude,

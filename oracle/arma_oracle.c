@@ -49,6 +49,10 @@
  *                       bisection (src/linalg.hh:199-213), and the identity
  *                       transforms (Gram-Charlier (0, 3) maps x to x/stdev,
  *                       skew normal alpha 0 to mean + sigma x / stdev).
+ *   orc_acf_generate    pinned: numpy restatement (numpy.fft auto-covariance,
+ *                       ddof=1 variance) and the circular auto-covariance by
+ *                       brute force on a tiny grid; acf(0,0,0) is the mean
+ *                       square of the decayed wave.
  *   orc_derivative_xy   pinned: exact on quadratics in x and y, raw slice for
  *                       extents below 3.
  *   orc_high_amplitude_potential
@@ -1251,4 +1255,142 @@ void orc_nit_realisation(double* data, size_t n, double stdev, int dist, const d
 		} while (i < max_iter && (b - a) > eps && fabs(fc) > eps);
 		data[q] = c;
 	}
+}
+
+/* ------------------------------------------------------------------------ */
+/* ACF_generator (src/generator/acf_generator.cc:130-215): the ACF as the    */
+/* auto-covariance of a synthetic plain wave with exponential decay.         */
+/* ------------------------------------------------------------------------ */
+
+/* Plain wave profiles (src/generator/plain_wave.hh): 0 sin, 1 cos, 2 stokes,
+   3 standing_wave; 2 pi written as the reference's T(2) * pi. */
+static double plain_wave(int func, double amp, double kx, double ky, double vel, double phase,
+                         double x, double y, double t) {
+	const double two_pi = 2.0 * M_PI;
+	switch (func) {
+	case 0: return amp * sin(two_pi * (kx * x + ky * y - vel * t) + phase);
+	case 1: return amp * sin(two_pi * (kx * x + ky * y - vel * t) + (phase + 0.5 * M_PI));
+	case 2: {
+		const double theta = two_pi * (kx * x + ky * y - vel * t) + phase;
+		const double st = two_pi * amp * sqrt(kx * kx + ky * ky);
+		return amp * (cos(theta) + 0.5 * st * cos(2.0 * theta) +
+		              (3.0 / 8.0) * st * st * cos(3.0 * theta));
+	}
+	default: return amp * cos(two_pi * (kx * x + ky * y) + phase) * cos(two_pi * (vel * t));
+	}
+}
+
+/* The wave on Domain(-r, r, ws + 1) (src/generator/acf_generator.hh:156-167),
+   t = -r0 + i (2 r0 / ws0) and so on. */
+static void wave_on_grid(int func, double amp, double vel, const double k[2], const double r[3],
+                         const int ws[3], double* out) {
+	const int n0 = ws[0] + 1, n1 = ws[1] + 1, n2 = ws[2] + 1;
+	double d[3];
+	for (int q = 0; q < 3; ++q) d[q] = ws[q] == 0 ? 0.0 : (r[q] - (-r[q])) / ws[q];
+	for (int i = 0; i < n0; ++i)
+		for (int j = 0; j < n1; ++j)
+			for (int l = 0; l < n2; ++l)
+				out[((size_t)i * n1 + j) * n2 + l] =
+					plain_wave(func, amp, k[0], k[1], vel, 0.0, -r[1] + d[1] * j, -r[2] + d[2] * l,
+					           -r[0] + d[0] * i);
+}
+
+/* auto_covariance (src/arma.cc:81-100): real(IFFT(|FFT(w)|^2)) / n / n. */
+static void auto_covariance(const double* w, const int s[3], double* out) {
+	const size_t n = (size_t)s[0] * s[1] * s[2];
+	double* re = (double*)malloc(sizeof(double) * n);
+	double* im = (double*)calloc(n, sizeof(double));
+	memcpy(re, w, sizeof(double) * n);
+	for (int sign = -1; sign <= 1; sign += 2) {
+		for (int j = 0; j < s[1]; ++j)
+			for (int l = 0; l < s[2]; ++l)
+				orc_fft_1d(re + (size_t)j * s[2] + l, im + (size_t)j * s[2] + l, s[0],
+				           s[1] * s[2], sign);
+		for (int i = 0; i < s[0]; ++i)
+			for (int l = 0; l < s[2]; ++l)
+				orc_fft_1d(re + (size_t)i * s[1] * s[2] + l, im + (size_t)i * s[1] * s[2] + l,
+				           s[1], s[2], sign);
+		for (int i = 0; i < s[0]; ++i)
+			for (int j = 0; j < s[1]; ++j)
+				orc_fft_1d(re + ((size_t)i * s[1] + j) * s[2], im + ((size_t)i * s[1] + j) * s[2],
+				           s[2], 1, sign);
+		if (sign < 0)
+			for (size_t q = 0; q < n; ++q) {
+				const double a = sqrt(re[q] * re[q] + im[q] * im[q]);
+				re[q] = a * a;
+				im[q] = 0.0;
+			}
+	}
+	for (size_t q = 0; q < n; ++q) out[q] = re[q] / (double)n / (double)n;
+	free(re);
+	free(im);
+}
+
+/* ACF_generator::generate().  func: 0 sin, 1 cos, 2 stokes, 3 standing_wave.
+   shape all > 0: the wave has 2 shape + 1 points per dimension; otherwise the
+   wave shape doubles from (2,2,2) while all extents are below 64 until the
+   variance changes by less than 1e-3.  out receives the first
+   (wave points / 2) per dimension of the auto-covariance (capacity cap);
+   shape_out that shape.  Returns 0, or -1 when cap is too small. */
+int orc_acf_generate(int func, double amplitude, double velocity, const double wavenum[2],
+                     const double alpha[3], const double nwaves[3], const int shape[3],
+                     double* out, size_t cap, int shape_out[3]) {
+	int ws[3] = {2, 2, 2};
+	double* wave = NULL;
+	size_t n = 0;
+	int s[3];
+	if (shape[0] > 0 && shape[1] > 0 && shape[2] > 0) {
+		for (int q = 0; q < 3; ++q) ws[q] = 2 * shape[q];
+		for (int q = 0; q < 3; ++q) s[q] = ws[q] + 1;
+		n = (size_t)s[0] * s[1] * s[2];
+		wave = (double*)malloc(sizeof(double) * n);
+		wave_on_grid(func, amplitude, velocity, wavenum, nwaves, ws, wave);
+	} else {
+		double var0 = -1.0, var = -1.0;
+		while (ws[0] < 64 && ws[1] < 64 && ws[2] < 64) {
+			for (int q = 0; q < 3; ++q) s[q] = ws[q] + 1;
+			n = (size_t)s[0] * s[1] * s[2];
+			free(wave);
+			wave = (double*)malloc(sizeof(double) * n);
+			wave_on_grid(func, amplitude, velocity, wavenum, nwaves, ws, wave);
+			var0 = var;
+			var = orc_gsl_variance_d(wave, n);
+			if (!(var0 < 0.0) && fabs(var - var0) < 1e-3) break;
+			for (int q = 0; q < 3; ++q) ws[q] *= 2;
+		}
+		for (int q = 0; q < 3; ++q) ws[q] = s[q] - 1;
+	}
+	/* add_exponential_decay (src/generator/acf_generator.cc:196-215) */
+	double d[3];
+	for (int q = 0; q < 3; ++q) d[q] = ws[q] == 0 ? 0.0 : (nwaves[q] - (-nwaves[q])) / ws[q];
+	double* we = (double*)malloc(sizeof(double) * n);
+	for (int i = 0; i < s[0]; ++i)
+		for (int j = 0; j < s[1]; ++j)
+			for (int l = 0; l < s[2]; ++l) {
+				const double t = -nwaves[0] + d[0] * i, x = -nwaves[1] + d[1] * j,
+				             y = -nwaves[2] + d[2] * l;
+				const size_t q = ((size_t)i * s[1] + j) * s[2] + l;
+				we[q] = wave[q] * exp(-(fabs(t * alpha[0]) + fabs(x * alpha[1]) +
+				                        fabs(y * alpha[2])));
+			}
+	const double f = sqrt(orc_gsl_variance_d(wave, n) / orc_gsl_variance_d(we, n));
+	for (size_t q = 0; q < n; ++q) we[q] *= f;
+	double* acf = (double*)malloc(sizeof(double) * n);
+	auto_covariance(we, s, acf);
+	int h[3];
+	for (int q = 0; q < 3; ++q) h[q] = s[q] / 2;
+	for (int q = 0; q < 3; ++q) shape_out[q] = h[q];
+	int rc = 0;
+	if ((size_t)h[0] * h[1] * h[2] > cap) {
+		rc = -1;
+	} else {
+		for (int i = 0; i < h[0]; ++i)
+			for (int j = 0; j < h[1]; ++j)
+				for (int l = 0; l < h[2]; ++l)
+					out[((size_t)i * h[1] + j) * h[2] + l] = acf[((size_t)i * s[1] + j) * s[2] + l];
+	}
+	free(wave);
+	free(we);
+	free(acf);
+	return rc;
 }

@@ -126,6 +126,12 @@ double orc_skew_normal_cdf(double x, double mean, double sigma, double alpha);
 void orc_nit_realisation(double* data, size_t n, double stdev, int dist, const double p[5],
                          double a0, double b0, double eps, int max_iter);
 
+/* ---- ACF_generator ---- */
+
+int orc_acf_generate(int func, double amplitude, double velocity, const double wavenum[2],
+                     const double alpha[3], const double nwaves[3], const int shape[3],
+                     double* out, size_t cap, int shape_out[3]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -323,3 +323,29 @@ def nit_realisation(data, stdev, dist, skewness=0.0, kurtosis=3.0, mean=0.0, sig
     _nit_lib().orc_nit_realisation(out.reshape(-1), out.size, stdev, dist, p, interval[0],
                                    interval[1], eps, max_iter)
     return out
+
+
+# ---- ACF_generator ----
+
+WAVE_SIN, WAVE_COS, WAVE_STOKES, WAVE_STANDING = 0, 1, 2, 3
+
+
+def acf_generate(func, amplitude=1.0, velocity=1.0, wavenum=(0.8, 0.0), alpha=(0.06,) * 3,
+                 nwaves=(1.5,) * 3, shape=(16, 16, 16)):
+    L = lib()
+    if not getattr(L, "_acfgen_set", False):
+        dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+        L.orc_acf_generate.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double * 2,
+                                       C.c_double * 3, C.c_double * 3, C.c_int * 3, dp,
+                                       C.c_size_t, C.c_int * 3]
+        L._acfgen_set = True
+    cap = 1 << 20
+    out = np.empty(cap)
+    so = (C.c_int * 3)()
+    rc = L.orc_acf_generate(func, amplitude, velocity, (C.c_double * 2)(*wavenum),
+                            (C.c_double * 3)(*alpha), (C.c_double * 3)(*nwaves),
+                            (C.c_int * 3)(*shape), out, cap, so)
+    if rc != 0:
+        raise ValueError("ACF too large")
+    s = tuple(so)
+    return out[:int(np.prod(s))].reshape(s).copy()

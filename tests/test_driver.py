@@ -160,3 +160,49 @@ def test_end_to_end_nit(tmp_path):
                             kurtosis=3.6, interval=(-5.0, 5.0))
     assert np.mean(np.abs(zeta - ref) <= 1e-9) >= 0.99
     assert np.max(np.abs(zeta - ref)) <= 1e-4
+
+
+HEAD_INPUT = """model = AR {
+	out_grid = (40,32,32)
+	acf = {
+		func = standing_wave
+		amplitude = 3
+		alpha = (2,0.2,1)
+		velocity = 0.50
+		beta = (0.0625,0)
+		nwaves = (1.85,16,1)
+		shape = (8,8,8)
+	}
+	algorithm = choi_recursive
+	order = (7,7,7)
+	output = surface
+	validate = 0
+}
+velocity_potential_solver = linear {
+	depth = 12
+	domain = from (10,-12) to (10,3) npoints (1,128)
+}
+"""
+
+
+def test_dry_run_acf_generator(tmp_path):
+    """An acf block without grid = selects ACF_generator (the reference's HEAD
+    syntax, input/standing_wave.arma with a smaller shape)."""
+    p = tmp_path / "head.arma"
+    p.write_text(HEAD_INPUT)
+    r = run(["-n", str(p)], str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    out = dict(line.split("=", 1) for line in r.stdout.splitlines())
+    acf = O.acf_generate(O.WAVE_STANDING, amplitude=3.0, velocity=0.5, wavenum=(0.0625, 0.0),
+                         alpha=(2.0, 0.2, 1.0), nwaves=(1.85, 16.0, 1.0), shape=(8, 8, 8))
+    phi, _ = O.yw_choi(acf, 7)
+    assert out["order"] == "(%d,%d,%d)" % phi.shape
+    got = np.array([float(v) for v in out["phi"].split(",")]).reshape(phi.shape)
+    # the two ACFs agree to 1e-11 (independent DFTs); the order-7 Choi solve
+    # amplifies that to about 1e-9 on this ACF
+    np.testing.assert_allclose(got, phi, rtol=1e-7, atol=1e-8)
+    # out_grid length = acf delta * patches * 2 pi (src/generator/basic_arma_model.cc:61-66),
+    # acf delta = nwaves / (8 - 1)
+    lengths = [float(v) for v in out["grid"].split(":")[1].strip("()").split(",")]
+    np.testing.assert_allclose(lengths, [1.85 / 7 * 39 * 2 * np.pi, 16 / 7 * 31 * 2 * np.pi,
+                                         1 / 7 * 31 * 2 * np.pi], rtol=1e-12)
