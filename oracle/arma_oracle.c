@@ -38,13 +38,16 @@
  *   orc_gauss_legendre_half, orc_owen_t
  *                       pinned: numpy's leggauss(20) nodes and weights;
  *                       T(0, a) = atan(a) / (2 pi), T(h, 1) =
- *                       Phi(h) (1 - Phi(h)) / 2, and adaptive quadrature.
+ *                       Phi(h) (1 - Phi(h)) / 2, adaptive quadrature, and
+ *                       the known answers of src/test/owen-t-test.cc.
  *   orc_gram_charlier_cdf, orc_skew_normal_cdf
  *                       pinned: Gram-Charlier with skewness 0 and kurtosis 3
  *                       is Phi; its derivative is the Gram-Charlier A density
  *                       phi (1 + s He3 / 6 + (k - 3) He4 / 24); skew normal
  *                       with alpha 0 is the normal CDF, its derivative the
- *                       skew normal density 2 phi(z) Phi(alpha z) / sigma.
+ *                       skew normal density 2 phi(z) Phi(alpha z) / sigma;
+ *                       skew normal at the nine known answers of
+ *                       src/test/skew-normal-test.cc.
  *   orc_nit_realisation pinned: brute-force Python restatement of the
  *                       bisection (src/linalg.hh:199-213), and the identity
  *                       transforms (Gram-Charlier (0, 3) maps x to x/stdev,

@@ -114,3 +114,17 @@ def test_library_acf_generate_defaults_and_doubling():
     assert got.shape == (16, 16, 16)
     with pytest.raises(ValueError):
         _abi.acf_generate(func=7)
+
+
+def test_reference_acf_generator_case():
+    """src/test/acf-generator-test.cc's input (it checks only that the
+    generator runs): cos, amplitude 4, alpha (2,1,10), velocity 1,
+    beta (0.4,0), nwaves 12.5, shape (32,1,1)."""
+    from arma_amd import _abi
+    args = dict(amplitude=4.0, velocity=1.0, alpha=(2.0, 1.0, 10.0), nwaves=(12.5,) * 3,
+                shape=(32, 1, 1))
+    got, delta = _abi.acf_generate(_abi.WAVE_COS, beta=(0.4, 0.0), **args)
+    ref = O.acf_generate(O.WAVE_COS, wavenum=(0.4, 0.0), **args)
+    assert got.shape == (32, 1, 1) and np.all(np.isfinite(got))
+    np.testing.assert_allclose(got, ref, rtol=1e-11, atol=1e-13 * np.abs(ref).max())
+    assert delta[0] == pytest.approx(12.5 / 31)

@@ -112,3 +112,23 @@ def test_nit_identity_transforms():
     sn = O.nit_realisation(data, sd, O.SKEW_NORMAL, mean=0.3, sigma=0.8, alpha=0.0,
                            interval=(-6.0, 6.0), eps=1e-12, max_iter=200)
     np.testing.assert_allclose(sn, 0.3 + 0.8 * data / sd, atol=1e-10)
+
+
+# ---- the reference's own known answers (src/test/owen-t-test.cc,
+# src/test/skew-normal-test.cc), at their tolerance 1e-5 ----
+
+def test_owen_t_reference_cases():
+    assert O.owen_t(1.0, 0.0) == pytest.approx(0.0, abs=1e-5)
+    assert O.owen_t(1.0, 1.0) == pytest.approx(O.owen_t(-1.0, 1.0), abs=1e-5)
+    assert O.owen_t(1.0, -1.0) == pytest.approx(-O.owen_t(1.0, 1.0), abs=1e-5)
+    for h, a, v in [(2.0, 0.5, 0.00862508), (1.0, 1.5, 0.07573), (1.5, 2.0, 0.0333832),
+                    (1.0, 0.1, 0.00960528)]:
+        assert O.owen_t(h, a) == pytest.approx(v, abs=1e-5)
+
+
+def test_skew_normal_reference_cases():
+    expect = [-1.0842021724855044e-19, 5.596621256709344e-13, 3.143618040497842e-7,
+              0.0017188799452888537, 0.14758361765043326, 0.6844083720823747,
+              0.9545000504654456, 0.9973002039372995, 0.9999366575163338]
+    for x, v in zip(range(-4, 5), expect):
+        assert O.skew_normal_cdf(float(x), 0.0, 1.0, 2.0) == pytest.approx(v, abs=1e-5)
