@@ -357,15 +357,11 @@ __device__ __forceinline__ void wave_tile(
 #pragma unroll
 		for (int d = 0; d < kD; ++d) step(u + d, qe[d], qh0[d], qh1[d], qt[d], ql[d]);
 	}
-	static_assert(kD == 4, "the tail below names every queue slot");
-	for (int d = 0; u < total_u; ++u, ++d) {
-		switch (d) {
-#define ARMA_TAIL(D) \
-	case D: step(u, qe[D], qh0[D], qh1[D], qt[D], ql[D]); break;
-			ARMA_TAIL(0) ARMA_TAIL(1) ARMA_TAIL(2) ARMA_TAIL(3)
-#undef ARMA_TAIL
-		}
-	}
+	// the last steps (fewer than kD), each on its own queue slot; the
+	// condition is the same on every lane
+#pragma unroll
+	for (int d = 0; d < kD; ++d)
+		if (u + d < total_u) step(u + d, qe[d], qh0[d], qh1[d], qt[d], ql[d]);
 	// every store complete, then the final count
 	__builtin_amdgcn_s_waitcnt(0);
 	__syncthreads();

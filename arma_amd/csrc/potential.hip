@@ -49,6 +49,7 @@
 
 #include <cmath>
 #include <cstdlib>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -521,6 +522,44 @@ struct WinParts {
 	}
 };
 
+// The reference replaces every non-finite window value by zero
+// (src/velocity/linear_solver.cc:176-183).  Here W only overflows, where
+// z > 0 and l z is large: above ~708 in FP64, where the reference's FP64
+// expression is non-finite too, and above ~88 in FP32, where the value does
+// not fit the working type (as the reference's FP32 expression).
+template <class T>
+__device__ __forceinline__ double finite_or_zero(double w) {
+	return isfinite(T(w)) ? w : 0.0;
+}
+
+// Symmetric and antisymmetric parts of the window at one level from
+// W_rot(k) = va and W_rot(-k) = vb.  CHECKED (a level range where W may
+// exceed the working type): non-finite values become zero first and the
+// halves are added (the sum of two values near the top of the range would
+// overflow).  Otherwise 0.5 (va +- vb): the same bits for any pair of
+// values below the top of the range.
+template <class T, bool CHECKED>
+__device__ __forceinline__ void win_pair(double va, double vb, T& w, T& wd) {
+	if constexpr (CHECKED) {
+		va = finite_or_zero<T>(va);
+		vb = finite_or_zero<T>(vb);
+		w = T(0.5 * va + 0.5 * vb);
+		wd = T(0.5 * va - 0.5 * vb);
+	} else {
+		w = T(0.5 * (va + vb));
+		wd = T(0.5 * (va - vb));
+	}
+}
+
+// Whether W may exceed the working type at some level z in [zlo, zhi]:
+// W is below 4 pi e^{l z} + 4 pi, so l z <= 80 keeps it under FLT_MAX and
+// l z <= 700 keeps it (and a sum of two) under DBL_MAX.
+template <class T>
+__device__ __forceinline__ bool window_may_overflow(double l, double zlo, double zhi) {
+	const double zmax = fmax(zlo, zhi);
+	return zmax > 0.0 && l * zmax > (sizeof(T) == 4 ? 80.0 : 700.0);
+}
+
 // ---- inverse transforms -------------------------------------------------------
 
 // Column task: packed columns c0..c0+CB-1 of slice i (column 0 carries
@@ -559,9 +598,13 @@ __device__ __forceinline__ void col_task(cpx<T>* X, int ny, int M, int nz, int i
 	const double wy = 1.0 / kmin[4 * i + 1];
 	const double du = wx / double(N - 1 > 1 ? N - 1 : 1);
 	const double dv = wy / double(ny - 1 > 1 ? ny - 1 : 1);
+	// every wave number of the slice is at most 2 pi |(wx, wy)|
+	const double lmax = kTwoPi * sqrt(wx * wx + wy * wy);
 	for (int j0 = 0; j0 < nz; j0 += ZB) {
 		const int zb = min(ZB, nz - j0);
 		const double zs = z0 + dz * j0;
+		// the same for the whole workgroup: one code path per level group
+		const bool checked = window_may_overflow<T>(lmax, zs, zs + dz * (zb - 1));
 		// F * W_sym for the levels of this group
 #pragma unroll
 		for (int e = 0; e < EPT; ++e) {
@@ -576,35 +619,47 @@ __device__ __forceinline__ void col_task(cpx<T>* X, int ny, int M, int nz, int i
 					wa.init(rot_index(kx, N), rot_index(ky, ny), du, dv, zs, dz, h, zb);
 					wb.init(rot_index(kxm, N), rot_index(ky == 0 ? 0 : ny - ky, ny), du, dv, zs,
 					        dz, h, zb);
-					for (int zz = 0; zz < zb; ++zz) {
-						const double va = wa.next(zz, zs, dz, h), vb = wb.next(zz, zs, dz, h);
-						const T w = T(0.5 * (va + vb));
-						if (ha) {
-							const T wd = T(0.5 * (va - vb));
-							dst[zz * CB * N] = cpx<T>{f0[e].x * w - g0[e].y * wd,
-							                          f0[e].y * w + g0[e].x * wd};
-						} else {
-							dst[zz * CB * N] = cpx<T>{f0[e].x * w, f0[e].y * w};
+					auto levels = [&](auto checked) {
+						for (int zz = 0; zz < zb; ++zz) {
+							T w, wd;
+							win_pair<T, decltype(checked)::value>(wa.next(zz, zs, dz, h),
+							                                      wb.next(zz, zs, dz, h), w, wd);
+							if (ha)
+								dst[zz * CB * N] = cpx<T>{f0[e].x * w - g0[e].y * wd,
+								                          f0[e].y * w + g0[e].x * wd};
+							else
+								dst[zz * CB * N] = cpx<T>{f0[e].x * w, f0[e].y * w};
 						}
-					}
+					};
+					if (checked)
+						levels(std::true_type{});
+					else
+						levels(std::false_type{});
 				}
 				if (ky == 0) {
 					// packed column: add i * F(kx, M) W_sym(kx, M)
 					WinParts wa, wb;
 					wa.init(rot_index(kx, N), rot_index(M, ny), du, dv, zs, dz, h, zb);
 					wb.init(rot_index(kxm, N), rot_index(M, ny), du, dv, zs, dz, h, zb);
-					for (int zz = 0; zz < zb; ++zz) {
-						const double va = wa.next(zz, zs, dz, h), vb = wb.next(zz, zs, dz, h);
-						const T w = T(0.5 * (va + vb));
-						const cpx<T> a = dst[zz * CB * N];
-						if (ha) {
-							const T wd = T(0.5 * (va - vb));
-							const cpx<T> hm{f1[e].x * w - g1[e].y * wd, f1[e].y * w + g1[e].x * wd};
-							dst[zz * CB * N] = cpx<T>{a.x - hm.y, a.y + hm.x};
-						} else {
-							dst[zz * CB * N] = cpx<T>{a.x - f1[e].y * w, a.y + f1[e].x * w};
+					auto levels = [&](auto checked) {
+						for (int zz = 0; zz < zb; ++zz) {
+							T w, wd;
+							win_pair<T, decltype(checked)::value>(wa.next(zz, zs, dz, h),
+							                                      wb.next(zz, zs, dz, h), w, wd);
+							const cpx<T> a = dst[zz * CB * N];
+							if (ha) {
+								const cpx<T> hm{f1[e].x * w - g1[e].y * wd,
+								                f1[e].y * w + g1[e].x * wd};
+								dst[zz * CB * N] = cpx<T>{a.x - hm.y, a.y + hm.x};
+							} else {
+								dst[zz * CB * N] = cpx<T>{a.x - f1[e].y * w, a.y + f1[e].x * w};
+							}
 						}
-					}
+					};
+					if (checked)
+						levels(std::true_type{});
+					else
+						levels(std::false_type{});
 				}
 			}
 		}

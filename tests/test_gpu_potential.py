@@ -149,3 +149,21 @@ def test_high_amplitude_host_pointers_and_errors(hip_ctx):
     assert np.max(np.abs(out - ref)) <= 1e-10 * np.max(np.abs(ref))
     with pytest.raises(ValueError):
         run_gpu(hip_ctx, zeta, 5.0, (0.0, -6.0), (3.0, 0.0), (4, 2), solver="high_amplitude")
+
+
+@pytest.mark.parametrize("solver", ["linear", "high_amplitude"])
+def test_potential_window_overflow(hip_ctx, solver):
+    """z up to 5 with wave numbers up to ~270: the window overflows at the
+    high wave numbers of the upper levels, where the reference zeroes it
+    (src/velocity/linear_solver.cc:176-183).  The surface is scaled by
+    1e-250 so that the surviving huge window values leave a finite field;
+    every (t, z) plane must match the oracle relative to its own maximum."""
+    zeta = ar_surface((6, 16, 16), 1) * 1e-250
+    npts, lb, ub = (6, 7), (0.0, -10.0), (5.0, 5.0)
+    got, _ = run_gpu(hip_ctx, zeta, 12.0, lb, ub, npts, solver=solver)
+    ref = O.linear_potential(zeta, (TWO_PI,) * 3, 12.0, lb, ub, npts, solver=solver)
+    assert np.isfinite(ref).all() and np.isfinite(got).all()
+    for i in range(npts[0]):
+        for j in range(npts[1]):
+            scale = np.max(np.abs(ref[i, j]))
+            assert np.max(np.abs(got[i, j] - ref[i, j])) <= 1e-9 * scale, (i, j)
