@@ -476,11 +476,19 @@ __device__ __forceinline__ int rot_index(int j, int n) {
 	return j;
 }
 
-// W at one wave number along nz equally spaced levels z_j = zs + j dz:
-// W_j = scale (a_j + c_j), a_j = e^{l z_j}, c_j = e^{-l (z_j + 2h)}.
+// W at one wave number along nz equally spaced levels z_j = zs + j dz, in
+// the working precision R (FP64, or FP32 as the reference's FP32 build
+// evaluates it): W_j = scale (a_j + c_j), a_j = e^{l z_j},
+// c_j = e^{-l (z_j + 2h)}.
+template <class R>
 struct WinParts {
-	double a, c, rho, irho, scale, l;
-	int jstart;   // a_j = 0 for j < jstart (e^{l z} below 1e-304)
+	// exponent below which a_j is taken as zero (e^x under 1e-304 in FP64,
+	// 1e-35 in FP32), and the step l |dz| above which every level is
+	// evaluated directly
+	static constexpr double kLow = sizeof(R) == 4 ? -80.0 : -700.0;
+	static constexpr double kDirect = sizeof(R) == 4 ? 60.0 : 600.0;
+	R a, c, rho, irho, scale, l;
+	int jstart;   // a_j = 0 for j < jstart
 	bool direct;  // step too large: evaluate the exponentials directly
 
 	__device__ __forceinline__ void init(int px, int py, double du, double dv, double zs,
@@ -489,31 +497,31 @@ struct WinParts {
 			px = 1;
 			py = 1;
 		}
-		const double u = du * px, v = dv * py;
-		l = kTwoPi * sqrt(u * u + v * v);
-		scale = kFourPi / (1.0 + exp(-2.0 * l * h));
-		direct = l * fabs(dz) > 600.0;
-		c = exp(-l * (zs + 2.0 * h));
-		rho = exp(l * dz);
-		irho = exp(-l * dz);
-		// first level where l z > -700
+		const R u = R(du * px), v = R(dv * py);
+		l = R(kTwoPi) * sqrt(u * u + v * v);
+		scale = R(kFourPi) / (R(1) + exp(R(-2) * l * R(h)));
+		direct = double(l) * fabs(dz) > kDirect;
+		c = exp(-l * R(zs + 2.0 * h));
+		rho = exp(l * R(dz));
+		irho = exp(-l * R(dz));
+		// first level where l z > kLow
 		int js = 0;
-		if (l * zs < -700.0 && dz > 0.0) {
-			js = int(ceil((-700.0 / l - zs) / dz));
+		if (double(l) * zs < kLow && dz > 0.0) {
+			js = int(ceil((kLow / double(l) - zs) / dz));
 			if (js < 0) js = 0;
 		}
 		jstart = js;
-		a = js < nz ? exp(l * (zs + dz * js)) : 0.0;
+		a = js < nz ? exp(l * R(zs + dz * js)) : R(0);
 	}
 	// W_j for j = 0, 1, 2, ... in order
-	__device__ __forceinline__ double next(int j, double zs, double dz, double h) {
-		double av, cv;
+	__device__ __forceinline__ R next(int j, double zs, double dz, double h) {
+		R av, cv;
 		if (direct) {
-			const double zj = zs + dz * j;
+			const R zj = R(zs + dz * j);
 			av = exp(l * zj);
-			cv = exp(-l * (zj + 2.0 * h));
+			cv = exp(-l * (zj + R(2.0 * h)));
 		} else {
-			av = j < jstart ? 0.0 : a;
+			av = j < jstart ? R(0) : a;
 			cv = c;
 			if (j >= jstart) a *= rho;
 			c *= irho;
@@ -579,6 +587,8 @@ __device__ __forceinline__ void col_task(cpx<T>* X, int ny, int M, int nz, int i
 	const int nel = CB * N;
 	const size_t sbase = size_t(i) * (M + 1) * N;
 	const bool ha = specb != nullptr;
+	// the window in the working precision (FP32 for FP32 transforms)
+	using WR = std::conditional_t<sizeof(T) == 4, float, double>;
 	cpx<T> f0[EPT], f1[EPT], g0[EPT], g1[EPT];
 #pragma unroll
 	for (int e = 0; e < EPT; ++e) {
@@ -615,7 +625,7 @@ __device__ __forceinline__ void col_task(cpx<T>* X, int ny, int M, int nz, int i
 				const int kxm = kx == 0 ? 0 : N - kx;
 				cpx<T>* dst = X + c * N + kx;  // level zz at dst[zz * CB * N]
 				{
-					WinParts wa, wb;
+					WinParts<WR> wa, wb;
 					wa.init(rot_index(kx, N), rot_index(ky, ny), du, dv, zs, dz, h, zb);
 					wb.init(rot_index(kxm, N), rot_index(ky == 0 ? 0 : ny - ky, ny), du, dv, zs,
 					        dz, h, zb);
@@ -638,7 +648,7 @@ __device__ __forceinline__ void col_task(cpx<T>* X, int ny, int M, int nz, int i
 				}
 				if (ky == 0) {
 					// packed column: add i * F(kx, M) W_sym(kx, M)
-					WinParts wa, wb;
+					WinParts<WR> wa, wb;
 					wa.init(rot_index(kx, N), rot_index(M, ny), du, dv, zs, dz, h, zb);
 					wb.init(rot_index(kxm, N), rot_index(M, ny), du, dv, zs, dz, h, zb);
 					auto levels = [&](auto checked) {
