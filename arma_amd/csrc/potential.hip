@@ -172,6 +172,39 @@ constexpr int xsize(int nseq) {
 	return nseq * (N2 * (N1 + 1) + 1);
 }
 
+// A complex value in LDS, aligned to its size so that it moves as one 8- or
+// 16-byte access (two 8-byte halves would each cover only half the banks).
+template <class R>
+struct alignas(2 * sizeof(R)) lds_cpx {
+	R x, y;
+	__device__ __forceinline__ lds_cpx& operator=(cpx<R> a) {
+		x = a.x;
+		y = a.y;
+		return *this;
+	}
+	__device__ __forceinline__ operator cpx<R>() const { return cpx<R>{x, y}; }
+};
+
+// In-place exchange index within one sequence's LDS row of the four-step
+// transform: output k2 of the first DFTs, input n1 of the second.  n1 is
+// XOR-swizzled with k2 so that both the writes (n1 varying across lanes) and
+// the reads (k2 varying) fall in distinct banks of a 256-byte bank row.
+template <int N1, class T>
+__device__ __forceinline__ int swz(int k2, int n1) {
+	constexpr int E = 256 / int(sizeof(cpx<T>));  // elements per bank row
+	constexpr int D = E / N1 > 1 ? E / N1 : 1;
+	return k2 * N1 + (n1 ^ ((k2 / D) & (N1 - 1)));
+}
+
+// Orders a wave's LDS stores before its later LDS loads of what other lanes
+// of the same wave stored (LDS operations of one wave complete in order;
+// this only keeps the compiler from moving them across).
+__device__ __forceinline__ void wave_lds_sync() {
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ---- forward spectrum -------------------------------------------------------
 
 // Rows of the real field of field_value (-zeta_t, or the plane src), real
@@ -184,8 +217,8 @@ __global__ __launch_bounds__(kThreads) void fwd_rows_kernel(
 	const T* __restrict__ src, const double2* __restrict__ twy, cpx<T>* __restrict__ spec,
 	int R) {
 	constexpr int M = N1 * N2;
-	extern __shared__ unsigned char smem_raw[];
-	cpx<T>* X = reinterpret_cast<cpx<T>*>(smem_raw);
+	extern __shared__ __align__(16) unsigned char smem_raw[];
+	lds_cpx<T>* X = reinterpret_cast<lds_cpx<T>*>(smem_raw);
 	const int i = blockIdx.y;
 	const int x0 = blockIdx.x * R;
 	const int it = idx_t[i];
@@ -238,7 +271,7 @@ __global__ __launch_bounds__(kThreads) void fwd_rows_kernel(
 		const int x = x0 + r;
 		if (x >= nx) continue;
 		const cpx<T> zk = X[r * (M + 1) + (k == M ? 0 : k)];
-		const cpx<T> zm = conj(X[r * (M + 1) + (k == 0 ? 0 : M - k)]);
+		const cpx<T> zm = conj(cpx<T>(X[r * (M + 1) + (k == 0 ? 0 : M - k)]));
 		const cpx<T> e = zk + zm;
 		const cpx<T> wd = twiddle<-1>(zk - zm, twy, k);
 		spec[(size_t(i) * (M + 1) + k) * nx + x] =
@@ -252,8 +285,8 @@ template <class T, int N1, int N2>
 __global__ __launch_bounds__(kThreads) void fwd_cols_kernel(int nx, int M,
                                                              const double2* __restrict__ twx,
                                                              cpx<T>* __restrict__ spec, int S) {
-	extern __shared__ unsigned char smem_raw[];
-	cpx<T>* X = reinterpret_cast<cpx<T>*>(smem_raw);
+	extern __shared__ __align__(16) unsigned char smem_raw[];
+	lds_cpx<T>* X = reinterpret_cast<lds_cpx<T>*>(smem_raw);
 	const int i = blockIdx.y;
 	const int c0 = blockIdx.x * S;
 	cpx<T>* base = spec + size_t(i) * (M + 1) * nx;
@@ -570,12 +603,18 @@ __device__ __forceinline__ bool window_may_overflow(double l, double zlo, double
 
 // ---- inverse transforms -------------------------------------------------------
 
+// Levels per seed of the window recurrence: W at level j is reached from the
+// exponentials at the last multiple of kWinSpan by at most kWinSpan - 1 steps.
+constexpr int kWinSpan = 16;
+
 // Column task: packed columns c0..c0+CB-1 of slice i (column 0 carries
 // ky = 0 and ky = M), times W_sym(z) for every level, inverse transform of
 // length nx = N1*N2 along x.  Output Gs[z][c][x].  Levels go in groups of ZB
-// through LDS; ZB*CB*N1 <= 256.
+// through LDS; ZB*CB*N1 <= 256, and ZB divides kWinSpan or covers all levels.
+// The recurrence state of the window runs on from one group to the next and is
+// seeded at every multiple of kWinSpan, so the values do not depend on ZB.
 template <class T, int N1, int N2>
-__device__ __forceinline__ void col_task(cpx<T>* X, int ny, int M, int nz, int i, int c0,
+__device__ __forceinline__ void col_task(lds_cpx<T>* X, int ny, int M, int nz, int i, int c0,
                                          const double2* __restrict__ twx,
                                          const cpx<T>* __restrict__ spec,
                                          const cpx<T>* __restrict__ specb,
@@ -610,9 +649,13 @@ __device__ __forceinline__ void col_task(cpx<T>* X, int ny, int M, int nz, int i
 	const double dv = wy / double(ny - 1 > 1 ? ny - 1 : 1);
 	// every wave number of the slice is at most 2 pi |(wx, wy)|
 	const double lmax = kTwoPi * sqrt(wx * wx + wy * wy);
+	WinParts<WR> wpa[EPT], wpb[EPT];  // the main column's window, per element
 	for (int j0 = 0; j0 < nz; j0 += ZB) {
 		const int zb = min(ZB, nz - j0);
 		const double zs = z0 + dz * j0;
+		// the seed of this group's levels
+		const int jseed = j0 - j0 % kWinSpan, span = min(kWinSpan, nz - jseed);
+		const double zseed = z0 + dz * jseed;
 		// the same for the whole workgroup: one code path per level group
 		const bool checked = window_may_overflow<T>(lmax, zs, zs + dz * (zb - 1));
 		// F * W_sym for the levels of this group
@@ -623,17 +666,22 @@ __device__ __forceinline__ void col_task(cpx<T>* X, int ny, int M, int nz, int i
 				const int c = q / N, kx = q % N;
 				const int ky = c0 + c;
 				const int kxm = kx == 0 ? 0 : N - kx;
-				cpx<T>* dst = X + c * N + kx;  // level zz at dst[zz * CB * N]
+				lds_cpx<T>* dst = X + c * N + kx;  // level zz at dst[zz * CB * N]
 				{
-					WinParts<WR> wa, wb;
-					wa.init(rot_index(kx, N), rot_index(ky, ny), du, dv, zs, dz, h, zb);
-					wb.init(rot_index(kxm, N), rot_index(ky == 0 ? 0 : ny - ky, ny), du, dv, zs,
-					        dz, h, zb);
+					WinParts<WR>& wa = wpa[e];
+					WinParts<WR>& wb = wpb[e];
+					if (j0 == jseed) {
+						wa.init(rot_index(kx, N), rot_index(ky, ny), du, dv, zseed, dz, h, span);
+						wb.init(rot_index(kxm, N), rot_index(ky == 0 ? 0 : ny - ky, ny), du, dv,
+						        zseed, dz, h, span);
+					}
+					const int jz = j0 - jseed;
 					auto levels = [&](auto checked) {
 						for (int zz = 0; zz < zb; ++zz) {
 							T w, wd;
-							win_pair<T, decltype(checked)::value>(wa.next(zz, zs, dz, h),
-							                                      wb.next(zz, zs, dz, h), w, wd);
+							win_pair<T, decltype(checked)::value>(
+								wa.next(jz + zz, zseed, dz, h), wb.next(jz + zz, zseed, dz, h), w,
+								wd);
 							if (ha)
 								dst[zz * CB * N] = cpx<T>{f0[e].x * w - g0[e].y * wd,
 								                          f0[e].y * w + g0[e].x * wd};
@@ -647,15 +695,23 @@ __device__ __forceinline__ void col_task(cpx<T>* X, int ny, int M, int nz, int i
 						levels(std::false_type{});
 				}
 				if (ky == 0) {
-					// packed column: add i * F(kx, M) W_sym(kx, M)
-					WinParts<WR> wa, wb;
-					wa.init(rot_index(kx, N), rot_index(M, ny), du, dv, zs, dz, h, zb);
-					wb.init(rot_index(kxm, N), rot_index(M, ny), du, dv, zs, dz, h, zb);
+					// packed column: add i * F(kx, M) W_sym(kx, M); its window is
+					// seeded again in every group and stepped to the group's first
+					// level (one workgroup in M / CB has this column)
+					WinParts<WR> pa, pb;
+					pa.init(rot_index(kx, N), rot_index(M, ny), du, dv, zseed, dz, h, span);
+					pb.init(rot_index(kxm, N), rot_index(M, ny), du, dv, zseed, dz, h, span);
+					const int jz = j0 - jseed;
+					for (int zz = 0; zz < jz; ++zz) {
+						pa.next(zz, zseed, dz, h);
+						pb.next(zz, zseed, dz, h);
+					}
 					auto levels = [&](auto checked) {
 						for (int zz = 0; zz < zb; ++zz) {
 							T w, wd;
-							win_pair<T, decltype(checked)::value>(wa.next(zz, zs, dz, h),
-							                                      wb.next(zz, zs, dz, h), w, wd);
+							win_pair<T, decltype(checked)::value>(
+								pa.next(jz + zz, zseed, dz, h), pb.next(jz + zz, zseed, dz, h), w,
+								wd);
 							const cpx<T> a = dst[zz * CB * N];
 							if (ha) {
 								const cpx<T> hm{f1[e].x * w - g1[e].y * wd,
@@ -674,33 +730,44 @@ __device__ __forceinline__ void col_task(cpx<T>* X, int ny, int M, int nz, int i
 			}
 		}
 		__syncthreads();
+		// Sequence s = tid / N1 lies in the LDS row X[s N .. s N + N - 1] and
+		// its N1 lanes belong to one wave (N1 divides 64), so from here on
+		// each wave transforms its own sequences in place, without a
+		// workgroup barrier.  The lanes' loads of a sequence are complete
+		// before any of them stores into it: every store depends on all of
+		// the loaded values.
 		const int nseq = zb * CB;
 		const bool act = tid < nseq * N1;
 		const int n1 = tid % N1, s = tid / N1;
-		cpx<T> v[N2];
 		if (act) {
+			cpx<T> v[N2];
 #pragma unroll
 			for (int n2 = 0; n2 < N2; ++n2) v[n2] = X[s * N + n1 + N1 * n2];
-		}
-		__syncthreads();
-		if (act) {
 			dft<N2, +1>(v);
 			twiddle_out<N2, +1>(v, twx, n1,
-			                    [&](int k2, cpx<T> a) { X[xidx<N1, N2>(s, k2, n1)] = a; });
+			                    [&](int k2, cpx<T> a) { X[s * N + swz<N1, T>(k2, n1)] = a; });
 		}
-		__syncthreads();
-		for (int t = tid; t < nseq * N2; t += kThreads) {
-			const int k2 = t % N2, sq = t / N2;
-			const int zz = sq / CB, c = sq % CB;
-			cpx<T> w[N1];
+		wave_lds_sync();
+		// the same wave's sequences, one (sequence, k2) per lane
+		constexpr int SPW = 64 / N1;  // sequences per wave
+		const int lane = tid % 64, seq0 = (tid / 64) * SPW;
 #pragma unroll
-			for (int m = 0; m < N1; ++m) w[m] = X[xidx<N1, N2>(sq, k2, m)];
-			dft<N1, +1>(w);
-			cpx<T>* out = Gs + (size_t(j0 + zz) * M + c0 + c) * N;
+		for (int it = 0; it < SPW * N2 / 64; ++it) {
+			const int u = lane + 64 * it;
+			const int sq = seq0 + u / N2, k2 = u % N2;
+			if (sq < nseq) {
+				const int zz = sq / CB, c = sq % CB;
+				cpx<T> w[N1];
 #pragma unroll
-			for (int m = 0; m < N1; ++m) out[k2 + N2 * brev(m, N1)] = w[m];
+				for (int m = 0; m < N1; ++m) w[m] = X[sq * N + swz<N1, T>(k2, m)];
+				dft<N1, +1>(w);
+				cpx<T>* out = Gs + (size_t(j0 + zz) * M + c0 + c) * N;
+#pragma unroll
+				for (int m = 0; m < N1; ++m) out[k2 + N2 * brev(m, N1)] = w[m];
+			}
 		}
-		__syncthreads();
+		// the next group's window step overwrites every row
+		if (j0 + ZB < nz) __syncthreads();
 	}
 }
 
@@ -708,7 +775,7 @@ __device__ __forceinline__ void col_task(cpx<T>* X, int ny, int M, int nz, int i
 // ny = 2M from the M packed columns g[k][x], through a complex transform of
 // length M = N1*N2, scaled by 1/(nx ny).  R*N1 <= 256.
 template <class T, int N1, int N2>
-__device__ __forceinline__ void row_task(cpx<T>* X, int nx, int ny, int x0,
+__device__ __forceinline__ void row_task(lds_cpx<T>* X, int nx, int ny, int x0,
                                          const double2* __restrict__ twy, const cpx<T>* g,
                                          T* __restrict__ out, int R) {
 	constexpr int M = N1 * N2;
@@ -741,7 +808,7 @@ __device__ __forceinline__ void row_task(cpx<T>* X, int nx, int ny, int x0,
 		for (int n2 = 0; n2 < N2; ++n2) {
 			const int k = n1 + N1 * n2;
 			// Z[k] = (H[k] + conj H[M-k]) + i e^{i pi k/M} (H[k] - conj H[M-k])
-			const cpx<T> a = hk[n2], b = conj(X[r * (M + 1) + (k == 0 ? 0 : M - k)]);
+			const cpx<T> a = hk[n2], b = conj(cpx<T>(X[r * (M + 1) + (k == 0 ? 0 : M - k)]));
 			const cpx<T> sum = a + b;
 			const cpx<T> wd = cmul(a - b, T(tk[n2].x), T(-tk[n2].y));
 			v[n2] = cpx<T>{sum.x - wd.y, sum.y + wd.x};
@@ -783,8 +850,8 @@ __global__ __launch_bounds__(kThreads) void inv_cols_kernel(
 	const cpx<T>* __restrict__ spec, const cpx<T>* __restrict__ specb,
 	const double* __restrict__ kmin, double z0, double dz, double h, cpx<T>* __restrict__ G,
 	int CB, int ZB) {
-	extern __shared__ unsigned char smem_raw[];
-	cpx<T>* X = reinterpret_cast<cpx<T>*>(smem_raw);
+	extern __shared__ __align__(16) unsigned char smem_raw[];
+	lds_cpx<T>* X = reinterpret_cast<lds_cpx<T>*>(smem_raw);
 	const int il = blockIdx.y;
 	col_task<T, N1, N2>(X, ny, M, nz, i0 + il, blockIdx.x * CB, twx, spec, specb, kmin, z0, dz,
 	                    h, G + size_t(il) * nz * M * (N1 * N2), CB, ZB);
@@ -797,8 +864,8 @@ __global__ __launch_bounds__(kThreads) void inv_rows_kernel(int nx, int ny, int 
                                                              const double2* __restrict__ twy,
                                                              const cpx<T>* __restrict__ G,
                                                              T* __restrict__ phi, int R) {
-	extern __shared__ unsigned char smem_raw[];
-	cpx<T>* X = reinterpret_cast<cpx<T>*>(smem_raw);
+	extern __shared__ __align__(16) unsigned char smem_raw[];
+	lds_cpx<T>* X = reinterpret_cast<lds_cpx<T>*>(smem_raw);
 	const int item = blockIdx.y;
 	const int il = item / nz, j = item - il * nz;
 	row_task<T, N1, N2>(X, nx, ny, blockIdx.x * R, twy, G + size_t(item) * (N1 * N2) * nx,
@@ -914,15 +981,27 @@ size_t chunk_bytes() {
 	return size_t(8) << 30;
 }
 
+// LDS bytes of a column-pass workgroup (ARMA_HIP_POT_COL_LDS_KB overrides).
+size_t col_lds_cap() {
+	if (const char* e = std::getenv("ARMA_HIP_POT_COL_LDS_KB")) {
+		const long v = std::atol(e);
+		if (v > 0) return size_t(v) << 10;
+	}
+	return 81920;
+}
+
 // Column and row passes for nx = NX and ny = 2M over chunks of output slices.
 template <class T, int NX, int M>
 void launch_inverse(arma_hip_ctx* ctx, const PotPlan& p, const double2* twx, const double2* twy,
                     const cpx<T>* spec, const double* dk, T* phi) {
 	constexpr int CN1 = split1(NX), CN2 = NX / CN1, RN1 = split1(M), RN2 = M / RN1;
 	const int CB = std::max(1, std::min(M, kThreads / NX));
-	const size_t per_seq = sizeof(cpx<T>) * size_t(std::max(NX, xsize<CN1, CN2>(1)));
-	int ZB = std::min(p.nz, std::max(1, kThreads / CN1 / CB));
-	while (ZB > 1 && size_t(ZB) * CB * per_seq > 81920) --ZB;
+	static_assert(CN1 <= 64 && CN2 >= CN1, "a wave holds whole sequences");
+	const size_t per_seq = sizeof(cpx<T>) * size_t(NX);
+	// a power of two that divides kWinSpan, or all the levels
+	int ZB = std::min(kWinSpan, std::max(1, kThreads / CN1 / CB));
+	while (ZB > 1 && size_t(ZB) * CB * per_seq > col_lds_cap()) ZB /= 2;
+	if (p.nz <= ZB) ZB = p.nz;
 	const size_t lds_c = size_t(ZB) * CB * per_seq;
 	const int R = std::min(NX, kThreads / RN1);
 	const size_t lds_r = sizeof(cpx<T>) * size_t(xsize<RN1, RN2>(R));
