@@ -611,8 +611,9 @@ constexpr int kWinSpan = 16;
 // ky = 0 and ky = M), times W_sym(z) for every level, inverse transform of
 // length nx = N1*N2 along x.  Output Gs[z][c][x].  Levels go in groups of ZB
 // through LDS; ZB*CB*N1 <= 256, and ZB divides kWinSpan or covers all levels.
-// The recurrence state of the window runs on from one group to the next and is
-// seeded at every multiple of kWinSpan, so the values do not depend on ZB.
+// For transforms of up to 2*256 points the recurrence state of the window runs
+// on from one group to the next and is seeded at every multiple of kWinSpan,
+// so the values do not depend on ZB; longer transforms seed every group.
 template <class T, int N1, int N2>
 __device__ __forceinline__ void col_task(lds_cpx<T>* X, int ny, int M, int nz, int i, int c0,
                                          const double2* __restrict__ twx,
@@ -653,8 +654,12 @@ __device__ __forceinline__ void col_task(lds_cpx<T>* X, int ny, int M, int nz, i
 	for (int j0 = 0; j0 < nz; j0 += ZB) {
 		const int zb = min(ZB, nz - j0);
 		const double zs = z0 + dz * j0;
-		// the seed of this group's levels
-		const int jseed = j0 - j0 % kWinSpan, span = min(kWinSpan, nz - jseed);
+		// the seed of this group's levels (with more than two elements per
+		// lane the carried state would not fit the registers: every group is
+		// seeded at its first level)
+		constexpr bool kCarry = EPT <= 2;
+		const int jseed = kCarry ? j0 - j0 % kWinSpan : j0;
+		const int span = kCarry ? min(kWinSpan, nz - jseed) : zb;
 		const double zseed = z0 + dz * jseed;
 		// the same for the whole workgroup: one code path per level group
 		const bool checked = window_may_overflow<T>(lmax, zs, zs + dz * (zb - 1));

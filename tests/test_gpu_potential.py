@@ -49,6 +49,23 @@ def test_potential_fp64_matches_oracle(hip_ctx, shape, npts, lb, ub):
     assert np.max(np.abs(got - ref)) <= 1e-10 * np.max(np.abs(ref))
 
 
+@pytest.mark.parametrize("shape,nz", [
+    ((2, 1024, 16), 20),  # four elements per lane: every level group seeds the window
+    ((3, 512, 32), 20),   # two per lane: the window runs on across groups of 8 levels
+    ((4, 64, 32), 40),    # seeds at levels 0, 16 and 32
+])
+@pytest.mark.parametrize("solver", ["linear", "high_amplitude"])
+def test_potential_long_transforms_and_many_levels(hip_ctx, shape, nz, solver):
+    print(f"potential {solver} {shape} nz={nz}", flush=True)
+    zeta = ar_surface(shape, 3)
+    npts, lb, ub = (shape[0], nz), (0.0, -10.0), (float(shape[0] - 1), 0.0)
+    delta = (TWO_PI, 0.5, 0.5)
+    got, _ = run_gpu(hip_ctx, zeta, 12.0, lb, ub, npts, solver=solver, delta=delta)
+    fn = O.linear_potential if solver == "linear" else O.high_amplitude_potential
+    ref = fn(zeta, delta, 12.0, lb, ub, npts)
+    assert np.max(np.abs(got - ref)) <= 1e-10 * np.max(np.abs(ref))
+
+
 def test_potential_fp32(hip_ctx):
     zeta = ar_surface((8, 64, 64), 2)
     npts, lb, ub = (8, 8), (0.0, -10.0), (7.0, 0.0)
