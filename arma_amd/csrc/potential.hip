@@ -343,6 +343,9 @@ __device__ __forceinline__ Ev first_max(const Ev& cur, const Ev& e) {
 	return fabs(cur.elev) < fabs(e.elev) ? e : cur;
 }
 
+// Positions whose loads a column scan issues together.
+constexpr int kFwAhead = 16;
+
 // Columns of the row tile staged in LDS by the row scan.
 __host__ __device__ constexpr int fw_tile(int lines) { return lines > 512 ? 8 : 16; }
 
@@ -412,11 +415,23 @@ __global__ __launch_bounds__(1024) void factor_waves_kernel(const T* __restrict_
 	if (dir == 0) {
 		if (n0 >= 3 && has_line) {
 			double zm = double(z[b]), zc = double(z[size_t(ny) + b]);
-			for (int a = 1; a + 1 < n0; ++a) {
-				const double zp = double(z[size_t(a + 1) * ny + b]);
-				visit(zm, zc, zp);
-				zm = zc;
-				zc = zp;
+			// the loads of kFwAhead positions are issued together, ahead of
+			// their (sequential) run updates
+			for (int a0 = 1; a0 + 1 < n0; a0 += kFwAhead) {
+				double zp[kFwAhead];
+#pragma unroll
+				for (int u = 0; u < kFwAhead; ++u) {
+					const int a = min(a0 + u + 1, n0 - 1);
+					zp[u] = double(z[size_t(a) * ny + b]);
+				}
+#pragma unroll
+				for (int u = 0; u < kFwAhead; ++u) {
+					if (a0 + u + 1 < n0) {
+						visit(zm, zc, zp[u]);
+						zm = zc;
+						zc = zp[u];
+					}
+				}
 			}
 		}
 	} else {
@@ -430,12 +445,25 @@ __global__ __launch_bounds__(1024) void factor_waves_kernel(const T* __restrict_
 			}
 			__syncthreads();
 			const int cw = has_line ? min(TW, n0 - c0) : 0;
-			for (int col = 0; col < cw; ++col) {
-				const double zp = tile[b * (TW + 1) + col];
-				const int c = c0 + col;  // position of zp along the row
-				if (c >= 2) visit(zm, zc, zp);
-				zm = zc;
-				zc = zp;
+			if (cw == 16) {
+				// a whole tile row: its LDS reads are issued together
+				double zr[16];
+#pragma unroll
+				for (int col = 0; col < 16; ++col) zr[col] = tile[b * (TW + 1) + col];
+#pragma unroll
+				for (int col = 0; col < 16; ++col) {
+					if (c0 + col >= 2) visit(zm, zc, zr[col]);
+					zm = zc;
+					zc = zr[col];
+				}
+			} else {
+				for (int col = 0; col < cw; ++col) {
+					const double zp = tile[b * (TW + 1) + col];
+					const int c = c0 + col;  // position of zp along the row
+					if (c >= 2) visit(zm, zc, zp);
+					zm = zc;
+					zc = zp;
+				}
 			}
 		}
 		__syncthreads();  // the tile area becomes the summary array
@@ -782,7 +810,7 @@ __device__ __forceinline__ void col_task(lds_cpx<T>* X, int ny, int M, int nz, i
 template <class T, int N1, int N2>
 __device__ __forceinline__ void row_task(lds_cpx<T>* X, int nx, int ny, int x0,
                                          const double2* __restrict__ twy, const cpx<T>* g,
-                                         T* __restrict__ out, int R) {
+                                         T* __restrict__ out, int R, T* E) {
 	constexpr int M = N1 * N2;
 	const int tid = threadIdx.x;
 	const bool act = tid < R * N1;
@@ -837,13 +865,88 @@ __device__ __forceinline__ void row_task(lds_cpx<T>* X, int nx, int ny, int x0,
 		for (int n1 = 0; n1 < N1; ++n1) w[n1] = X[xidx<N1, N2>(r, k2, n1)];
 		dft<N1, +1>(w);
 		if (x < nx) {
+			// column 0 is left to the edge fix-up; columns 1 and 2 of the row
+			// also go to E for it
+			if (k2 == 0) {
+				out[size_t(x) * ny + 1] = w[0].y * inv;
+				E[2 * r] = w[0].y * inv;
+			} else {
+				*reinterpret_cast<cpx<T>*>(out + size_t(x) * ny + 2 * k2) =
+					cpx<T>{w[0].x * inv, w[0].y * inv};
+				if (k2 == 1) E[2 * r + 1] = w[0].x * inv;
+			}
 #pragma unroll
-			for (int m = 0; m < N1; ++m) {
+			for (int m = 1; m < N1; ++m) {
 				const int n = k2 + N2 * brev(m, N1);
 				*reinterpret_cast<cpx<T>*>(out + size_t(x) * ny + 2 * n) =
 					cpx<T>{w[m].x * inv, w[m].y * inv};
 			}
 		}
+	}
+}
+
+// Plane through three grid points evaluated at a fourth
+// (src/interpolate.cc:3-29, src/interpolate.hh:17-33).
+__device__ double interp3v(double fa, double fb, double fc, int a0, int a1, int b0, int b1,
+                           int c0, int c1, int x0, int x1) {
+	const double den = double(b0 * a1 - c0 * a1 - a0 * b1 + c0 * b1 + a0 * c1 - b0 * c1);
+	const double p0 = (fb * a1 - fc * a1 - fa * b1 + fc * b1 + fa * c1 - fb * c1) / den;
+	const double p1 = (fc * (-a0 + b0) + fb * (a0 - c0) + fa * (-b0 + c0)) /
+	                  double(c0 * (a1 - b1) + a0 * (b1 - c1) + b0 * (-a1 + c1));
+	const double p2 = (fc * b0 * a1 - fb * c0 * a1 - fc * a0 * b1 + fa * c0 * b1 +
+	                   fb * a0 * c1 - fa * b0 * c1) / den;
+	return p0 * x0 + p1 * x1 + p2;
+}
+
+template <class T>
+__device__ double interp3(const T* f, int ny, int a0, int a1, int b0, int b1, int c0, int c1,
+                          int x0, int x1) {
+	return interp3v(double(f[size_t(a0) * ny + a1]), double(f[size_t(b0) * ny + b1]),
+	                double(f[size_t(c0) * ny + c1]), a0, a1, b0, b1, c0, c1, x0, x1);
+}
+
+// Edge fix-up of src/velocity/basic_solver.cc:61-68 on one (slice, level)
+// plane f: (0,0), then column 0 (row k from columns 1 and 2 of rows k-1 and
+// k), then row 0 (from rows 1 and 2, and for l = 1 the fixed column values),
+// each point a plane through three neighbours.  Columns 1.. of rows 1.. never
+// change, so the column-0 points do not depend on each other, and row 0 reads
+// column 0 only at rows 1 and 2.
+//
+// The row pass stores no column 0.  Each of its workgroups fixes column 0 of
+// its rows x0+1..x0+R-1 (R >= 4) from columns 1 and 2 of its own rows, which
+// it keeps in LDS (E[2r], E[2r+1]).  edge_boundary_kernel then fixes the
+// first row x0 of every other workgroup (the row above is another
+// workgroup's), (0,0), and row 0 from rows 1 and 2, whose column 0 the pass
+// has fixed.
+template <class T>
+__device__ __forceinline__ void edge_rows(T* f, int nx, int ny, int x0, int R, const T* E) {
+	const int x1 = min(x0 + R, nx);
+	for (int k = x0 + 1 + int(threadIdx.x); k < x1; k += blockDim.x) {
+		const int r = k - x0;
+		f[size_t(k) * ny] = T(interp3v(double(E[2 * (r - 1)]), double(E[2 * r]),
+		                               double(E[2 * (r - 1) + 1]), k - 1, 1, k, 1, k - 1, 2, k,
+		                               0));
+	}
+}
+
+// One thread per point of each plane: column 0 of rows R, 2R, ..., then
+// (0,0), then row 0 at l = 1..ny-1.
+template <class T>
+__global__ void edge_boundary_kernel(int nx, int ny, T* __restrict__ phi, int nplanes, int R) {
+	const int nb = (nx - 1) / R;     // first rows of the row workgroups after the first
+	const int per = nb + ny;         // points per plane
+	const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (q >= (long long)nplanes * per) return;
+	const int plane = int(q / per), u = int(q % per);
+	T* f = phi + size_t(plane) * nx * ny;
+	if (u < nb) {
+		const int k = (u + 1) * R;
+		f[size_t(k) * ny] = T(interp3(f, ny, k - 1, 1, k, 1, k - 1, 2, k, 0));
+	} else if (u == nb) {
+		f[0] = T(interp3(f, ny, 1, 1, 1, 2, 2, 1, 0, 0));
+	} else {
+		const int l = u - nb;
+		f[l] = T(interp3(f, ny, 1, l - 1, 1, l, 2, l - 1, 0, l));
 	}
 }
 
@@ -873,41 +976,15 @@ __global__ __launch_bounds__(kThreads) void inv_rows_kernel(int nx, int ny, int 
 	lds_cpx<T>* X = reinterpret_cast<lds_cpx<T>*>(smem_raw);
 	const int item = blockIdx.y;
 	const int il = item / nz, j = item - il * nz;
-	row_task<T, N1, N2>(X, nx, ny, blockIdx.x * R, twy, G + size_t(item) * (N1 * N2) * nx,
-	                    phi + ((size_t(i0 + il) * nz + j) * nx) * ny, R);
-}
-
-// Plane through three grid points evaluated at a fourth
-// (src/interpolate.cc:3-29, src/interpolate.hh:17-33).
-template <class T>
-__device__ double interp3(const T* f, int ny, int a0, int a1, int b0, int b1, int c0, int c1,
-                          int x0, int x1) {
-	const double fa = double(f[size_t(a0) * ny + a1]);
-	const double fb = double(f[size_t(b0) * ny + b1]);
-	const double fc = double(f[size_t(c0) * ny + c1]);
-	const double den = double(b0 * a1 - c0 * a1 - a0 * b1 + c0 * b1 + a0 * c1 - b0 * c1);
-	const double p0 = (fb * a1 - fc * a1 - fa * b1 + fc * b1 + fa * c1 - fb * c1) / den;
-	const double p1 = (fc * (-a0 + b0) + fb * (a0 - c0) + fa * (-b0 + c0)) /
-	                  double(c0 * (a1 - b1) + a0 * (b1 - c1) + b0 * (-a1 + c1));
-	const double p2 = (fc * b0 * a1 - fb * c0 * a1 - fc * a0 * b1 + fa * c0 * b1 +
-	                   fb * a0 * c1 - fa * b0 * c1) / den;
-	return p0 * x0 + p1 * x1 + p2;
-}
-
-// Edge fix-up of src/velocity/basic_solver.cc:61-68: (0,0), then column 0
-// (reads columns 1 and 2 only), then row 0 (reads rows 1 and 2, and for
-// l = 1 the fixed column values).
-template <class T>
-__global__ void edge_kernel(int nx, int ny, T* __restrict__ phi, int i0, int nz) {
-	const int item = blockIdx.x;
-	const int il = item / nz, j = item - il * nz;
+	const int x0 = blockIdx.x * R;
 	T* f = phi + ((size_t(i0 + il) * nz + j) * nx) * ny;
-	if (threadIdx.x == 0) f[0] = T(interp3(f, ny, 1, 1, 1, 2, 2, 1, 0, 0));
-	for (int k = 1 + threadIdx.x; k < nx; k += blockDim.x)
-		f[size_t(k) * ny] = T(interp3(f, ny, k - 1, 1, k, 1, k - 1, 2, k, 0));
-	__syncthreads();
-	for (int l = 1 + threadIdx.x; l < ny; l += blockDim.x)
-		f[l] = T(interp3(f, ny, 1, l - 1, 1, l, 2, l - 1, 0, l));
+	T* E = reinterpret_cast<T*>(X + xsize<N1, N2>(R));  // columns 1 and 2 of the rows
+	row_task<T, N1, N2>(X, nx, ny, x0, twy, G + size_t(item) * (N1 * N2) * nx, f, R, E);
+	// E complete (an LDS-only barrier: the rows' global stores need not be)
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+	__builtin_amdgcn_s_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+	edge_rows(f, nx, ny, x0, R, E);
 }
 
 // ---- launch plumbing ------------------------------------------------------------
@@ -932,6 +1009,9 @@ struct PotPlan {
 	int solver;       // 0 linear, 1 high amplitude
 	int nfields() const { return solver == 1 ? 2 : 1; }
 };
+
+// Rows per row-pass workgroup for nx rows of ny = 2M: R * split1(M) lanes.
+int rows_per_task(int nx, int M) { return std::min(nx, kThreads / split1(M)); }
 
 // Length-dependent launches, instantiated for every power of two.
 template <class T, int NX>
@@ -986,14 +1066,10 @@ size_t chunk_bytes() {
 	return size_t(8) << 30;
 }
 
-// LDS bytes of a column-pass workgroup (ARMA_HIP_POT_COL_LDS_KB overrides).
-size_t col_lds_cap() {
-	if (const char* e = std::getenv("ARMA_HIP_POT_COL_LDS_KB")) {
-		const long v = std::atol(e);
-		if (v > 0) return size_t(v) << 10;
-	}
-	return 81920;
-}
+// LDS bytes of a column-pass workgroup at most.  Fewer levels per group
+// (less LDS, more resident workgroups) measured slower: the transform steps
+// then leave lanes idle.
+constexpr size_t kColLdsCap = 81920;
 
 // Column and row passes for nx = NX and ny = 2M over chunks of output slices.
 template <class T, int NX, int M>
@@ -1005,11 +1081,11 @@ void launch_inverse(arma_hip_ctx* ctx, const PotPlan& p, const double2* twx, con
 	const size_t per_seq = sizeof(cpx<T>) * size_t(NX);
 	// a power of two that divides kWinSpan, or all the levels
 	int ZB = std::min(kWinSpan, std::max(1, kThreads / CN1 / CB));
-	while (ZB > 1 && size_t(ZB) * CB * per_seq > col_lds_cap()) ZB /= 2;
+	while (ZB > 1 && size_t(ZB) * CB * per_seq > kColLdsCap) ZB /= 2;
 	if (p.nz <= ZB) ZB = p.nz;
 	const size_t lds_c = size_t(ZB) * CB * per_seq;
-	const int R = std::min(NX, kThreads / RN1);
-	const size_t lds_r = sizeof(cpx<T>) * size_t(xsize<RN1, RN2>(R));
+	const int R = rows_per_task(NX, M);
+	const size_t lds_r = sizeof(cpx<T>) * size_t(xsize<RN1, RN2>(R)) + sizeof(T) * 2 * R;
 	allow_lds(&inv_cols_kernel<T, CN1, CN2>, lds_c);
 	allow_lds(&inv_rows_kernel<T, RN1, RN2>, lds_r);
 	// the intermediate of a whole chunk round-trips through HBM; chunks are
@@ -1171,7 +1247,11 @@ void run_potential(arma_hip_ctx* ctx, const arma_pot_desc* d, const T* zeta, T* 
 	                    (const cpx<T>*)spec, (const double*)dk, phi);
 	{
 		KernelTimer t_ed(ctx, "pot_edge");
-		edge_kernel<T><<<p.nto * p.nz, 256, 0, st>>>(p.nx, p.ny, phi, 0, p.nz);
+		const int R = rows_per_task(p.nx, p.M);
+		const long long n = (long long)p.nto * p.nz * ((p.nx - 1) / R + p.ny);
+		if (n > 0)
+			edge_boundary_kernel<T><<<int((n + 255) / 256), 256, 0, st>>>(p.nx, p.ny, phi,
+			                                                              p.nto * p.nz, R);
 		ARMA_HIP_CHECK(hipGetLastError());
 	}
 	if (wnmax_out) {
