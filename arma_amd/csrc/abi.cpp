@@ -164,6 +164,7 @@ void arma_hip_destroy(arma_hip_ctx* ctx) {
 	for (arma_hip::Scratch* s : {&ctx->yw, &ctx->params, &ctx->seeds, &ctx->state, &ctx->phi,
 	                             &ctx->partials, &ctx->flags, &ctx->noise_tmp, &ctx->mbox, &ctx->pot_spec,
 	                             &ctx->pot_mid, &ctx->pot_wn, &ctx->pot_twiddle, &ctx->pot_kmin, &ctx->pot_field,
+	                             &ctx->pot_edge,
 	                             &ctx->zeta_tmp, &ctx->phi_tmp, &ctx->writer_stage})
 		s->release();
 	if (ctx->writer_host) (void)hipHostFree(ctx->writer_host);

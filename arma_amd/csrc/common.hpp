@@ -89,7 +89,8 @@ struct arma_hip_ctx {
 	int num_cus = 0;
 	// named scratch areas
 	arma_hip::Scratch params, seeds, state, phi, partials, flags, noise_tmp, mbox;
-	arma_hip::Scratch pot_spec, pot_mid, pot_wn, pot_twiddle, pot_kmin, pot_field, zeta_tmp, phi_tmp;
+	arma_hip::Scratch pot_spec, pot_mid, pot_wn, pot_twiddle, pot_kmin, pot_field, pot_edge,
+	    zeta_tmp, phi_tmp;
 	arma_hip::Scratch yw;
 	int twiddle_n = 0;
 	// batch configurations chosen by the autotuner, by batch signature:

@@ -810,7 +810,7 @@ __device__ __forceinline__ void col_task(lds_cpx<T>* X, int ny, int M, int nz, i
 template <class T, int N1, int N2>
 __device__ __forceinline__ void row_task(lds_cpx<T>* X, int nx, int ny, int x0,
                                          const double2* __restrict__ twy, const cpx<T>* g,
-                                         T* __restrict__ out, int R, T* E) {
+                                         T* __restrict__ out, int R, T* __restrict__ E) {
 	constexpr int M = N1 * N2;
 	const int tid = threadIdx.x;
 	const bool act = tid < R * N1;
@@ -865,18 +865,11 @@ __device__ __forceinline__ void row_task(lds_cpx<T>* X, int nx, int ny, int x0,
 		for (int n1 = 0; n1 < N1; ++n1) w[n1] = X[xidx<N1, N2>(r, k2, n1)];
 		dft<N1, +1>(w);
 		if (x < nx) {
-			// column 0 is left to the edge fix-up; columns 1 and 2 of the row
-			// also go to E for it
-			if (k2 == 0) {
-				out[size_t(x) * ny + 1] = w[0].y * inv;
-				E[2 * r] = w[0].y * inv;
-			} else {
-				*reinterpret_cast<cpx<T>*>(out + size_t(x) * ny + 2 * k2) =
-					cpx<T>{w[0].x * inv, w[0].y * inv};
-				if (k2 == 1) E[2 * r + 1] = w[0].x * inv;
-			}
+			// columns 1 and 2 of the row also go to the edge fix-up's array
+			if (k2 == 0) E[2 * size_t(x)] = w[0].y * inv;
+			if (k2 == 1) E[2 * size_t(x) + 1] = w[0].x * inv;
 #pragma unroll
-			for (int m = 1; m < N1; ++m) {
+			for (int m = 0; m < N1; ++m) {
 				const int n = k2 + N2 * brev(m, N1);
 				*reinterpret_cast<cpx<T>*>(out + size_t(x) * ny + 2 * n) =
 					cpx<T>{w[m].x * inv, w[m].y * inv};
@@ -909,44 +902,39 @@ __device__ double interp3(const T* f, int ny, int a0, int a1, int b0, int b1, in
 // plane f: (0,0), then column 0 (row k from columns 1 and 2 of rows k-1 and
 // k), then row 0 (from rows 1 and 2, and for l = 1 the fixed column values),
 // each point a plane through three neighbours.  Columns 1.. of rows 1.. never
-// change, so the column-0 points do not depend on each other, and row 0 reads
-// column 0 only at rows 1 and 2.
-//
-// The row pass stores no column 0.  Each of its workgroups fixes column 0 of
-// its rows x0+1..x0+R-1 (R >= 4) from columns 1 and 2 of its own rows, which
-// it keeps in LDS (E[2r], E[2r+1]).  edge_boundary_kernel then fixes the
-// first row x0 of every other workgroup (the row above is another
-// workgroup's), (0,0), and row 0 from rows 1 and 2, whose column 0 the pass
-// has fixed.
+// change.  The row pass writes the transformed plane in full and, beside it,
+// columns 1 and 2 of every row contiguously (E[x][0..1]); this kernel (one
+// thread per point: column 0 at k = 1..nx-1, (0,0), row 0 at l = 1..ny-1)
+// takes column 1 and 2 values from E only, so the column points read no
+// value the kernel writes, and row 0 at l = 1 evaluates the fixed (1,0) and
+// (2,0) itself.
 template <class T>
-__device__ __forceinline__ void edge_rows(T* f, int nx, int ny, int x0, int R, const T* E) {
-	const int x1 = min(x0 + R, nx);
-	for (int k = x0 + 1 + int(threadIdx.x); k < x1; k += blockDim.x) {
-		const int r = k - x0;
-		f[size_t(k) * ny] = T(interp3v(double(E[2 * (r - 1)]), double(E[2 * r]),
-		                               double(E[2 * (r - 1) + 1]), k - 1, 1, k, 1, k - 1, 2, k,
-		                               0));
-	}
-}
-
-// One thread per point of each plane: column 0 of rows R, 2R, ..., then
-// (0,0), then row 0 at l = 1..ny-1.
-template <class T>
-__global__ void edge_boundary_kernel(int nx, int ny, T* __restrict__ phi, int nplanes, int R) {
-	const int nb = (nx - 1) / R;     // first rows of the row workgroups after the first
-	const int per = nb + ny;         // points per plane
+__global__ void edge_kernel(int nx, int ny, T* __restrict__ phi, const T* __restrict__ edge,
+                            int nplanes) {
+	const int per = nx + ny - 1;  // points per plane
 	const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (q >= (long long)nplanes * per) return;
 	const int plane = int(q / per), u = int(q % per);
 	T* f = phi + size_t(plane) * nx * ny;
-	if (u < nb) {
-		const int k = (u + 1) * R;
-		f[size_t(k) * ny] = T(interp3(f, ny, k - 1, 1, k, 1, k - 1, 2, k, 0));
-	} else if (u == nb) {
-		f[0] = T(interp3(f, ny, 1, 1, 1, 2, 2, 1, 0, 0));
+	const T* E = edge + size_t(plane) * nx * 2;
+	// column 0 at row k from E
+	auto col0 = [&](int k) {
+		return interp3v(double(E[2 * (k - 1)]), double(E[2 * k]), double(E[2 * (k - 1) + 1]),
+		                k - 1, 1, k, 1, k - 1, 2, k, 0);
+	};
+	if (u < nx - 1) {
+		const int k = u + 1;
+		f[size_t(k) * ny] = T(col0(k));
+	} else if (u == nx - 1) {
+		f[0] = T(interp3v(double(E[2]), double(E[3]), double(E[4]), 1, 1, 1, 2, 2, 1, 0, 0));
 	} else {
-		const int l = u - nb;
-		f[l] = T(interp3(f, ny, 1, l - 1, 1, l, 2, l - 1, 0, l));
+		const int l = u - (nx - 1);
+		if (l == 1) {
+			const double f10 = double(T(col0(1))), f20 = double(T(col0(2)));
+			f[1] = T(interp3v(f10, double(E[2]), f20, 1, 0, 1, 1, 2, 0, 0, 1));
+		} else {
+			f[l] = T(interp3(f, ny, 1, l - 1, 1, l, 2, l - 1, 0, l));
+		}
 	}
 }
 
@@ -971,20 +959,16 @@ template <class T, int N1, int N2>
 __global__ __launch_bounds__(kThreads) void inv_rows_kernel(int nx, int ny, int nz, int i0,
                                                              const double2* __restrict__ twy,
                                                              const cpx<T>* __restrict__ G,
-                                                             T* __restrict__ phi, int R) {
+                                                             T* __restrict__ phi,
+                                                             T* __restrict__ edge, int R) {
 	extern __shared__ __align__(16) unsigned char smem_raw[];
 	lds_cpx<T>* X = reinterpret_cast<lds_cpx<T>*>(smem_raw);
 	const int item = blockIdx.y;
 	const int il = item / nz, j = item - il * nz;
 	const int x0 = blockIdx.x * R;
-	T* f = phi + ((size_t(i0 + il) * nz + j) * nx) * ny;
-	T* E = reinterpret_cast<T*>(X + xsize<N1, N2>(R));  // columns 1 and 2 of the rows
-	row_task<T, N1, N2>(X, nx, ny, x0, twy, G + size_t(item) * (N1 * N2) * nx, f, R, E);
-	// E complete (an LDS-only barrier: the rows' global stores need not be)
-	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-	__builtin_amdgcn_s_barrier();
-	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-	edge_rows(f, nx, ny, x0, R, E);
+	const size_t plane = size_t(i0 + il) * nz + j;
+	row_task<T, N1, N2>(X, nx, ny, x0, twy, G + size_t(item) * (N1 * N2) * nx,
+	                    phi + plane * nx * ny, R, edge + plane * nx * 2);
 }
 
 // ---- launch plumbing ------------------------------------------------------------
@@ -1074,7 +1058,7 @@ constexpr size_t kColLdsCap = 81920;
 // Column and row passes for nx = NX and ny = 2M over chunks of output slices.
 template <class T, int NX, int M>
 void launch_inverse(arma_hip_ctx* ctx, const PotPlan& p, const double2* twx, const double2* twy,
-                    const cpx<T>* spec, const double* dk, T* phi) {
+                    const cpx<T>* spec, const double* dk, T* phi, T* edge) {
 	constexpr int CN1 = split1(NX), CN2 = NX / CN1, RN1 = split1(M), RN2 = M / RN1;
 	const int CB = std::max(1, std::min(M, kThreads / NX));
 	static_assert(CN1 <= 64 && CN2 >= CN1, "a wave holds whole sequences");
@@ -1085,7 +1069,7 @@ void launch_inverse(arma_hip_ctx* ctx, const PotPlan& p, const double2* twx, con
 	if (p.nz <= ZB) ZB = p.nz;
 	const size_t lds_c = size_t(ZB) * CB * per_seq;
 	const int R = rows_per_task(NX, M);
-	const size_t lds_r = sizeof(cpx<T>) * size_t(xsize<RN1, RN2>(R)) + sizeof(T) * 2 * R;
+	const size_t lds_r = sizeof(cpx<T>) * size_t(xsize<RN1, RN2>(R));
 	allow_lds(&inv_cols_kernel<T, CN1, CN2>, lds_c);
 	allow_lds(&inv_rows_kernel<T, RN1, RN2>, lds_r);
 	// the intermediate of a whole chunk round-trips through HBM; chunks are
@@ -1106,7 +1090,8 @@ void launch_inverse(arma_hip_ctx* ctx, const PotPlan& p, const double2* twx, con
 		{
 			KernelTimer t(ctx, "pot_inv_rows");
 			inv_rows_kernel<T, RN1, RN2><<<dim3(div_ceil(NX, R), cn * p.nz), kThreads, lds_r,
-			                               ctx->stream>>>(NX, 2 * M, p.nz, i0, twy, G, phi, R);
+			                               ctx->stream>>>(NX, 2 * M, p.nz, i0, twy, G, phi, edge,
+			                                              R);
 			ARMA_HIP_CHECK(hipGetLastError());
 		}
 	}
@@ -1114,13 +1099,13 @@ void launch_inverse(arma_hip_ctx* ctx, const PotPlan& p, const double2* twx, con
 
 template <class T, int NX, int M = 2>
 void inverse_by_m(int m, arma_hip_ctx* ctx, const PotPlan& p, const double2* twx,
-                  const double2* twy, const cpx<T>* spec, const double* dk, T* phi) {
+                  const double2* twy, const cpx<T>* spec, const double* dk, T* phi, T* edge) {
 	if constexpr (M <= 512) {
 		if (m == M) {
-			launch_inverse<T, NX, M>(ctx, p, twx, twy, spec, dk, phi);
+			launch_inverse<T, NX, M>(ctx, p, twx, twy, spec, dk, phi, edge);
 			return;
 		}
-		inverse_by_m<T, NX, M * 2>(m, ctx, p, twx, twy, spec, dk, phi);
+		inverse_by_m<T, NX, M * 2>(m, ctx, p, twx, twy, spec, dk, phi, edge);
 	} else {
 		ARMA_HIP_REQUIRE(false, ARMA_HIP_EINVAL, "transform length must be a power of two");
 	}
@@ -1128,13 +1113,14 @@ void inverse_by_m(int m, arma_hip_ctx* ctx, const PotPlan& p, const double2* twx
 
 template <class T, int NX = 4>
 void inverse_dispatch(int nx, int m, arma_hip_ctx* ctx, const PotPlan& p, const double2* twx,
-                      const double2* twy, const cpx<T>* spec, const double* dk, T* phi) {
+                      const double2* twy, const cpx<T>* spec, const double* dk, T* phi,
+                      T* edge) {
 	if constexpr (NX <= 1024) {
 		if (nx == NX) {
-			inverse_by_m<T, NX>(m, ctx, p, twx, twy, spec, dk, phi);
+			inverse_by_m<T, NX>(m, ctx, p, twx, twy, spec, dk, phi, edge);
 			return;
 		}
-		inverse_dispatch<T, NX * 2>(nx, m, ctx, p, twx, twy, spec, dk, phi);
+		inverse_dispatch<T, NX * 2>(nx, m, ctx, p, twx, twy, spec, dk, phi, edge);
 	} else {
 		ARMA_HIP_REQUIRE(false, ARMA_HIP_EINVAL, "transform length must be a power of two");
 	}
@@ -1243,15 +1229,14 @@ void run_potential(arma_hip_ctx* ctx, const arma_pot_desc* d, const T* zeta, T* 
 		by_length<Ops<T>::template ColFwd>(p.nx, ctx, p, (const double2*)twx, spec);
 	}
 	// every (slice, level): window, inverse along x, inverse along y
+	// columns 1 and 2 of every row of every plane, for the edge fix-up
+	T* edge = (T*)ctx->pot_edge.get(sizeof(T) * 2 * size_t(p.nto) * p.nz * p.nx);
 	inverse_dispatch<T>(p.nx, p.M, ctx, p, (const double2*)twx, (const double2*)twy,
-	                    (const cpx<T>*)spec, (const double*)dk, phi);
+	                    (const cpx<T>*)spec, (const double*)dk, phi, edge);
 	{
 		KernelTimer t_ed(ctx, "pot_edge");
-		const int R = rows_per_task(p.nx, p.M);
-		const long long n = (long long)p.nto * p.nz * ((p.nx - 1) / R + p.ny);
-		if (n > 0)
-			edge_boundary_kernel<T><<<int((n + 255) / 256), 256, 0, st>>>(p.nx, p.ny, phi,
-			                                                              p.nto * p.nz, R);
+		const long long n = (long long)p.nto * p.nz * (p.nx + p.ny - 1);
+		edge_kernel<T><<<int((n + 255) / 256), 256, 0, st>>>(p.nx, p.ny, phi, edge, p.nto * p.nz);
 		ARMA_HIP_CHECK(hipGetLastError());
 	}
 	if (wnmax_out) {

@@ -53,6 +53,8 @@ def test_potential_fp64_matches_oracle(hip_ctx, shape, npts, lb, ub):
     ((2, 1024, 16), 20),  # four elements per lane: every level group seeds the window
     ((3, 512, 32), 20),   # two per lane: the window runs on across groups of 8 levels
     ((4, 64, 32), 40),    # seeds at levels 0, 16 and 32
+    ((2, 64, 256), 3),    # edge fix-up over two row workgroups of 32 rows
+    ((2, 16, 1024), 3),   # rows of 1024 (32-point first DFTs in the row pass)
 ])
 @pytest.mark.parametrize("solver", ["linear", "high_amplitude"])
 def test_potential_long_transforms_and_many_levels(hip_ctx, shape, nz, solver):
