@@ -457,16 +457,28 @@ __global__ __launch_bounds__(64) void noise_seg_kernel(
 	const long long e12 = (long long)blk.e1 * e2;
 	const long long n = valid ? (long long)blk.e0 * e12 : 0;
 	const double r12 = 1.0 / double(e12), r2 = 1.0 / double(e2);
+	// block extents that are powers of two (full partitions of the usual
+	// shapes) split a position with shifts and masks
+	const bool pow2 = (e2 & (e2 - 1)) == 0 && (e12 & (e12 - 1)) == 0;
+	const int s2 = __ffs(unsigned(e2)) - 1, s12 = __ffsll((unsigned long long)e12) - 1;
 	auto put = [&](long long q, OutT v) {
-		// q < 2^31: 32-bit quotients through a double reciprocal, corrected
+		// q < 2^31: 32-bit quotients
 		const uint32_t uq = uint32_t(q), u12 = uint32_t(e12), u2 = uint32_t(e2);
-		uint32_t qt = uint32_t(double(uq) * r12);
-		if (uint64_t(qt) * u12 > uq) --qt;
-		else if (uint64_t(qt + 1) * u12 <= uq) ++qt;
-		const uint32_t rr = uq - qt * u12;
-		uint32_t qx = uint32_t(double(rr) * r2);
-		if (uint64_t(qx) * u2 > rr) --qx;
-		else if (uint64_t(qx + 1) * u2 <= rr) ++qx;
+		uint32_t qt, rr, qx;
+		if (pow2) {
+			qt = uq >> s12;
+			rr = uq & (u12 - 1u);
+			qx = rr >> s2;
+		} else {
+			// through a double reciprocal, corrected
+			qt = uint32_t(double(uq) * r12);
+			if (uint64_t(qt) * u12 > uq) --qt;
+			else if (uint64_t(qt + 1) * u12 <= uq) ++qt;
+			rr = uq - qt * u12;
+			qx = uint32_t(double(rr) * r2);
+			if (uint64_t(qx) * u2 > rr) --qx;
+			else if (uint64_t(qx + 1) * u2 <= rr) ++qx;
+		}
 		out[(size_t(lo0 + qt) * sy + (lo1 + qx)) * sz + (lo2 + (rr - qx * u2))] = v;
 	};
 	if (lane == 0) {
