@@ -40,10 +40,10 @@
 //    and B are formed by the real-row path, and only the window step of
 //    the column pass differs.
 //  * factor_waves is a sequential run scan per slice; it runs as a chunked
-//    scan where each thread summarises a contiguous chunk of events and one
-//    thread folds the summaries in order (the fold is exact: representatives
-//    are first-maximum picks, pairs are scored only between adjacent closed
-//    runs).
+//    scan where each thread summarises a contiguous chunk of events and the
+//    summaries are concatenated as a tree (exactly: representatives are
+//    first-maximum picks, pairs are scored only between adjacent closed
+//    runs, and the minimum and maximum of a set do not depend on the order).
 #include "common.hpp"
 #include "fft_reg.hpp"
 
@@ -343,14 +343,93 @@ __device__ __forceinline__ Ev first_max(const Ev& cur, const Ev& e) {
 	return fabs(cur.elev) < fabs(e.elev) ? e : cur;
 }
 
+// Summary of the concatenation of chunk a and the chunk b after it.  A
+// summary scores the pairs (r, r-1) of adjacent runs for 3 <= r < runs: its
+// last run is still open, and run 1 may merge with the run before the chunk,
+// so the pair (2, 1) is left to the caller.  The runs of a and b join where
+// a's last and b's first run have the same class (the representative is the
+// first maximum of the two), and the pairs that involve a's last or b's first
+// two runs are scored here when their index in the concatenation is at
+// least 3.  The scored pairs, and so the minimum and maximum, are those of a
+// sequential scan of the concatenation.
+__device__ __forceinline__ Summary concat(const Summary& a, const Summary& b) {
+	if (a.runs == 0) return b;
+	if (b.runs == 0) return a;
+	const int na = a.runs, nb = b.runs;
+	const bool merge = a.last_pos == b.first_pos;
+	Summary r;
+	r.runs = na + nb - (merge ? 1 : 0);
+	r.first_pos = a.first_pos;
+	r.last_pos = b.last_pos;
+	r.kmin = fmin(a.kmin, b.kmin);
+	r.kmax = fmax(a.kmax, b.kmax);
+	auto score = [&](double p1, double p2) {
+		const double k = pair_k(Ev{p1, 0.0}, Ev{p2, 0.0});
+		r.kmin = fmin(r.kmin, k);
+		r.kmax = fmax(r.kmax, k);
+	};
+	if (merge) {
+		// runs: a_1 .. a_(na-1), J, b_2 .. b_nb
+		const bool jb = fabs(a.last.elev) < fabs(b.first.elev);  // J = first_max(a.last, b.first)
+		const double jp = jb ? b.first.period : a.last.period;
+		const double je = jb ? b.first.elev : a.last.elev;
+		if (na >= 3 && nb >= 2) score(jp, a.prev_last.period);  // pair at index na
+		if (na >= 2 && nb >= 3) score(b.second.period, jp);     // pair at index na + 1
+		if (na >= 2) {
+			r.first = a.first;
+		} else {
+			r.first.period = jp;
+			r.first.elev = je;
+		}
+		if (na >= 3) {
+			r.second = a.second;
+		} else if (na == 2) {
+			r.second.period = jp;
+			r.second.elev = je;
+		} else {
+			r.second = b.second;
+		}
+		if (nb >= 2) {
+			r.last = b.last;
+		} else {
+			r.last.period = jp;
+			r.last.elev = je;
+		}
+		if (nb >= 3) {
+			r.prev_last = b.prev_last;
+		} else if (nb == 2) {
+			r.prev_last.period = jp;
+			r.prev_last.elev = je;
+		} else {
+			r.prev_last = a.prev_last;
+		}
+	} else {
+		// runs: a_1 .. a_na, b_1 .. b_nb
+		if (na >= 3) score(a.last.period, a.prev_last.period);     // index na
+		if (na >= 2 && nb >= 2) score(b.first.period, a.last.period);  // index na + 1
+		if (nb >= 3) score(b.second.period, b.first.period);       // index na + 2
+		r.first = a.first;
+		if (na >= 2)
+			r.second = a.second;
+		else
+			r.second = b.first;
+		r.last = b.last;
+		if (nb >= 2)
+			r.prev_last = b.prev_last;
+		else
+			r.prev_last = a.last;
+	}
+	return r;
+}
+
 // Positions whose loads a column scan issues together.
 constexpr int kFwAhead = 16;
 
 // Columns of the row tile staged in LDS by the row scan.
 __host__ __device__ constexpr int fw_tile(int lines) { return lines > 512 ? 8 : 16; }
 
-// One block per (slice, direction) with one thread per scanned line: thread
-// b scans line b (dir 0: column y = b along x; dir 1: row x = b along y), so
+// One block per (slice, direction) with one thread per scanned line (the
+// block size, a power of two, is the larger line count): thread b scans line b (dir 0: column y = b along x; dir 1: row x = b along y), so
 // its positions are exactly the contiguous chunk [b*inner, (b+1)*inner) of
 // the sequential scan order.  Column scans read coalesced rows of the slice;
 // row scans go through LDS tiles of the slice's columns.
@@ -372,6 +451,8 @@ __global__ __launch_bounds__(1024) void factor_waves_kernel(const T* __restrict_
 	const bool has_line = b < n1;  // the block spans the larger line count
 	Summary s;
 	s.runs = 0;
+	s.first_pos = s.last_pos = 0;
+	s.first = s.second = s.prev_last = s.last = Ev{0, 0};
 	s.kmin = 1.7976931348623157e308;
 	s.kmax = 2.2250738585072014e-308;
 	Ev cur{0, 0}, prev{0, 0};
@@ -473,54 +554,21 @@ __global__ __launch_bounds__(1024) void factor_waves_kernel(const T* __restrict_
 	s.last = cur;
 	s.last_pos = cur_pos;
 	s.prev_last = prev;
+	// the summaries of the lines, in scan order, combined as a tree
 	sm[threadIdx.x] = s;
-	__syncthreads();
+	for (int w = 1; w < int(blockDim.x); w *= 2) {
+		__syncthreads();
+		const int t = threadIdx.x;
+		if ((t & (2 * w - 1)) == 0 && t + w < int(blockDim.x)) sm[t] = concat(sm[t], sm[t + w]);
+	}
 	if (threadIdx.x == 0) {
-		bool have = false;
-		int cls = 0, j = 0;
-		Ev e1{0, 0}, e2{0, 0};
-		double kmin = 1.7976931348623157e308, kmax = 2.2250738585072014e-308;
-		auto transition = [&](const Ev& nxt, int ncls) {
-			if (j > 0) {
-				const double k = pair_k(e1, e2);
-				kmin = fmin(kmin, k);
-				kmax = fmax(kmax, k);
-			}
-			e2 = e1;
-			e1 = nxt;
-			cls = ncls;
-			++j;
-		};
-		for (int c = 0; c < n1; ++c) {
-			const Summary& q = sm[c];
-			if (q.runs == 0) continue;
-			if (!have) {
-				have = true;
-				cls = q.first_pos;
-				e1 = q.first;
-			} else if (q.first_pos == cls) {
-				e1 = first_max(e1, q.first);
-			} else {
-				transition(q.first, q.first_pos);
-			}
-			if (q.runs == 2) {
-				transition(q.last, q.last_pos);
-			} else if (q.runs >= 3) {
-				transition(q.second, 1 - cls);
-				if (q.runs == 3) {
-					transition(q.last, q.last_pos);
-				} else {
-					// entering run 3 scores (run 2, merged run 1); the pairs
-					// (3,2) .. (runs-1, runs-2) were scored inside the chunk
-					const double k = pair_k(e1, e2);
-					kmin = fmin(fmin(kmin, k), q.kmin);
-					kmax = fmax(fmax(kmax, k), q.kmax);
-					e2 = q.prev_last;
-					e1 = q.last;
-					cls = q.last_pos;
-					j += q.runs - 2;
-				}
-			}
+		const Summary& q = sm[0];
+		double kmin = q.kmin, kmax = q.kmax;
+		// the pair (run 2, run 1), which a summary leaves to its caller
+		if (q.runs >= 3) {
+			const double k = pair_k(q.second, q.first);
+			kmin = fmin(kmin, k);
+			kmax = fmax(kmax, k);
 		}
 		kout[4 * i + dir] = kmin;
 		kout[4 * i + 2 + dir] = kmax;
