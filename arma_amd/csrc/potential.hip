@@ -207,11 +207,54 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 // ---- forward spectrum -------------------------------------------------------
 
+// The field values at (x, 2n) and (x, 2n + 1), each as field_value computes
+// it; with VEC (zeta and src aligned to two elements) every slice is read
+// with one two-element load.
+template <class T, bool VEC>
+__device__ __forceinline__ cpx<T> field_pair(const T* __restrict__ zeta, int nt, int nx, int ny,
+                                             int it, int x, int n, double dt,
+                                             const T* __restrict__ src, int i) {
+	if constexpr (!VEC) {
+		return cpx<T>{T(field_value(zeta, nt, nx, ny, it, x, 2 * n, dt, src, i)),
+		              T(field_value(zeta, nt, nx, ny, it, x, 2 * n + 1, dt, src, i))};
+	} else {
+		using V = std::conditional_t<sizeof(T) == 8, double2, float2>;
+		const size_t slice = size_t(nx) * ny;
+		const size_t off = size_t(x) * ny + 2 * size_t(n);
+		auto ld = [](const T* p) { return *reinterpret_cast<const V*>(p); };
+		if (src) {
+			const V v = ld(src + size_t(i) * slice + off);
+			return cpx<T>{v.x, v.y};
+		}
+		// neg_dzdt on both components
+		const T* f = zeta + size_t(it) * slice + off;
+		double rx, ry;
+		if (nt - 1 < 2) {
+			const V a = ld(f);
+			rx = double(a.x);
+			ry = double(a.y);
+		} else if (it == 0) {
+			const V c2 = ld(f + 2 * slice), c1 = ld(f + slice), c0 = ld(f);
+			rx = 0.5 * (-double(c2.x) + 4.0 * double(c1.x) - 3.0 * double(c0.x)) / dt;
+			ry = 0.5 * (-double(c2.y) + 4.0 * double(c1.y) - 3.0 * double(c0.y)) / dt;
+		} else if (it == nt - 1) {
+			const V c0 = ld(f), m1 = ld(f - slice), m2 = ld(f - 2 * slice);
+			rx = 0.5 * (3.0 * double(c0.x) - 4.0 * double(m1.x) + double(m2.x)) / dt;
+			ry = 0.5 * (3.0 * double(c0.y) - 4.0 * double(m1.y) + double(m2.y)) / dt;
+		} else {
+			const V p1 = ld(f + slice), m1 = ld(f - slice);
+			rx = 0.5 * (double(p1.x) - double(m1.x)) / dt;
+			ry = 0.5 * (double(p1.y) - double(m1.y)) / dt;
+		}
+		return cpx<T>{T(-rx), T(-ry)};
+	}
+}
+
 // Rows of the real field of field_value (-zeta_t, or the plane src), real
 // transform of length ny = 2M along y via a complex transform of length
 // M = N1*N2.  Writes the half spectrum k = 0..M column-major: spec[i][k][x].
 // R rows per block, R*N1 <= 256.
-template <class T, int N1, int N2>
+template <class T, int N1, int N2, bool VEC>
 __global__ __launch_bounds__(kThreads) void fwd_rows_kernel(
 	const T* __restrict__ zeta, int nt, int nx, int ny, const int* __restrict__ idx_t, double dt,
 	const T* __restrict__ src, const double2* __restrict__ twy, cpx<T>* __restrict__ spec,
@@ -232,8 +275,7 @@ __global__ __launch_bounds__(kThreads) void fwd_rows_kernel(
 			const int n = n1 + N1 * n2;
 			v[n2] = cpx<T>{T(0), T(0)};
 			if (x < nx) {
-				v[n2] = cpx<T>{T(field_value(zeta, nt, nx, ny, it, x, 2 * n, dt, src, i)),
-				               T(field_value(zeta, nt, nx, ny, it, x, 2 * n + 1, dt, src, i))};
+				v[n2] = field_pair<T, VEC>(zeta, nt, nx, ny, it, x, n, dt, src, i);
 			}
 		}
 		dft<N2, -1>(v);
@@ -1072,14 +1114,18 @@ struct RowLaunch {
 	                    const double2* twy, cpx<T>* spec, const T* field) {
 		const int R = rows(p);
 		const size_t lds = sizeof(cpx<T>) * size_t(std::max(xsize<N1, N2>(R), R * (M + 1)));
-		allow_lds(&fwd_rows_kernel<T, N1, N2>, lds);
+		// two-element loads where the field's base allows them (a surface may
+		// start at any element of a caller's buffer)
+		const bool vec = reinterpret_cast<uintptr_t>(zeta) % (2 * sizeof(T)) == 0 &&
+		                 reinterpret_cast<uintptr_t>(field) % (2 * sizeof(T)) == 0;
+		auto kernel = vec ? &fwd_rows_kernel<T, N1, N2, true> : &fwd_rows_kernel<T, N1, N2, false>;
+		allow_lds(kernel, lds);
 		// linear: -zeta_t; high amplitude: the planes Re F, then Im F, into
 		// consecutive blocks of spec
 		const size_t sblock = size_t(p.nto) * (M + 1) * p.nx;
 		const size_t fblock = size_t(p.nto) * p.nx * p.ny;
 		for (int f = 0; f < p.nfields(); ++f) {
-			fwd_rows_kernel<T, N1, N2><<<dim3(div_ceil(p.nx, R), p.nto), kThreads, lds,
-			                             ctx->stream>>>(
+			kernel<<<dim3(div_ceil(p.nx, R), p.nto), kThreads, lds, ctx->stream>>>(
 				zeta, p.nt, p.nx, p.ny, didx, p.dt, field ? field + f * fblock : nullptr, twy,
 				spec + f * sblock, R);
 			ARMA_HIP_CHECK(hipGetLastError());

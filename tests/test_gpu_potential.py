@@ -105,6 +105,26 @@ def test_potential_host_pointers(hip_ctx):
     assert np.max(np.abs(out - ref)) <= 1e-10 * np.max(np.abs(ref))
 
 
+@pytest.mark.parametrize("precision", [arma_amd.FP64, arma_amd.FP32])
+def test_potential_unaligned_surface(hip_ctx, precision):
+    """A surface that starts at an odd element of its buffer (as in a batch
+    arena): the forward transform reads it element by element."""
+    dt = torch.float64 if precision == arma_amd.FP64 else torch.float32
+    zeta = ar_surface((4, 16, 16), 8)
+    npts = (4, 3)
+    buf = torch.zeros(zeta.size + 1, dtype=dt, device=DEV)
+    z = buf[1:].view(zeta.shape)
+    z.copy_(torch.from_numpy(zeta))
+    assert z.data_ptr() % (2 * z.element_size()) != 0
+    out = torch.empty((4, 3, 16, 16), dtype=dt, device=DEV)
+    d = hip_ctx.pot_desc(zeta.shape, (TWO_PI,) * 3, 12.0, (0.0, -5.0), (3.0, 0.0), npts, precision)
+    hip_ctx.linear_potential(d, z, out)
+    aligned, _ = run_gpu(hip_ctx, z.cpu().numpy().astype(np.float64), 12.0, (0.0, -5.0),
+                         (3.0, 0.0), npts, precision=precision)
+    # the same values as from an aligned copy, bit for bit
+    np.testing.assert_array_equal(out.cpu().numpy().astype(np.float64), aligned)
+
+
 def test_potential_errors(hip_ctx):
     zeta = ar_surface((4, 16, 16), 4)
     with pytest.raises(ValueError):
