@@ -562,9 +562,23 @@ __global__ __launch_bounds__(1024) void factor_waves_kernel(const T* __restrict_
 		double zm = 0.0, zc = 0.0;
 		for (int c0 = 0; c0 < n0; c0 += TW) {
 			__syncthreads();
-			for (int e = b; e < n1 * TW; e += blockDim.x) {
-				const int row = e / TW, col = e - row * TW;
-				if (c0 + col < n0) tile[row * (TW + 1) + col] = double(z[size_t(row) * ny + c0 + col]);
+			// a thread's loads of the tile (at most TW <= 16: n1 <= blockDim.x)
+			// are issued eight at a time, then stored
+#pragma unroll 1
+			for (int u0 = 0; u0 < 16; u0 += 8) {
+				T tv[8];
+#pragma unroll
+				for (int u = 0; u < 8; ++u) {
+					const int e = b + (u0 + u) * int(blockDim.x);
+					const int row = e / TW, col = e - row * TW;
+					tv[u] = e < n1 * TW && c0 + col < n0 ? z[size_t(row) * ny + c0 + col] : T(0);
+				}
+#pragma unroll
+				for (int u = 0; u < 8; ++u) {
+					const int e = b + (u0 + u) * int(blockDim.x);
+					const int row = e / TW, col = e - row * TW;
+					if (e < n1 * TW && c0 + col < n0) tile[row * (TW + 1) + col] = double(tv[u]);
+				}
 			}
 			__syncthreads();
 			const int cw = has_line ? min(TW, n0 - c0) : 0;
