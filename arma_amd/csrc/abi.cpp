@@ -606,6 +606,7 @@ int arma_hip_batch_tune(arma_hip_ctx* ctx, arma_hip_batch* batch, void* arena, i
 					arma_hip::ar_batch_launch(ctx, ab, arena);
 					ARMA_HIP_CHECK(hipEventRecord(e1, ctx->stream));
 					ARMA_HIP_CHECK(hipEventSynchronize(e1));
+					arma_hip::ar_batch_check(ctx, ab);
 					float t = 0.0f;
 					ARMA_HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
 					ms = std::min(ms, double(t));
@@ -676,6 +677,7 @@ int arma_hip_batch_factors(arma_hip_ctx* ctx, arma_hip_batch* batch, double* out
 		use_device(ctx);
 		ARMA_HIP_REQUIRE(batch && out, ARMA_HIP_EINVAL, "null argument");
 		arma_hip::rescale_batch_factors(ctx, batch->rescale, out);
+		arma_hip::ar_batch_check(ctx, batch->ar);
 	});
 }
 

@@ -40,6 +40,9 @@
 #include "common.hpp"
 
 #include <algorithm>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <vector>
 
 namespace arma_hip {
@@ -87,12 +90,30 @@ __global__ void reset_progress(int* p, int n) {
 }
 
 // Spin until *p >= need and return the value seen: relaxed device-coherent
-// polls, with an acquire load every 64 polls.
-__device__ __forceinline__ int wait_progress(const int* p, int need) {
+// polls, with an acquire load every 64 polls.  With limit > 0 the wait is
+// bounded: every 256 polls it gives up (returning need) once *abortw is set,
+// or sets *abortw itself when it has spun for more than limit ticks of the
+// constant-rate wall clock.  A launch that gives up still runs every tile to
+// its end; the host reads *abortw afterwards and reports the launch as
+// failed.
+__device__ __forceinline__ int wait_progress(const int* p, int need, int* abortw,
+                                             long long limit) {
+	long long start = 0;
 	for (int n = 1;; ++n) {
 		const int v = (n & 63) ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
 		                       : __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
 		if (v >= need) return v;
+		if (limit > 0 && (n & 255) == 0) {
+			if (__hip_atomic_load(abortw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+				return need;
+			const long long now = wall_clock64();
+			if (start == 0) {
+				start = now;
+			} else if (now - start > limit) {
+				__hip_atomic_store(abortw, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+				return need;
+			}
+		}
 		__builtin_amdgcn_s_sleep(1);
 	}
 }
@@ -147,7 +168,8 @@ __device__ __forceinline__ void wave_tile(
 	T* __restrict__ zeta, int nt, int nx, int ny, const T* __restrict__ phi0,
 	const T* __restrict__ phic, const int* __restrict__ toff, int P1, int P2, int nY,
 	int* __restrict__ progress, const T* __restrict__ zero, T* __restrict__ sink,
-	T* __restrict__ mbox, int NB, int total_steps, int t0, int b, unsigned char* smem_raw) {
+	T* __restrict__ mbox, int NB, int total_steps, int t0, int b, int* abortw, long long limit,
+	unsigned char* smem_raw) {
 	constexpr int kTX = TX, kTY = kThreads / TX;
 	static_assert(kTX * kTY == kThreads && kTX >= 8 && kTY >= 8, "tile shape");
 	const int P12 = P1 * P2;
@@ -267,8 +289,9 @@ __device__ __forceinline__ void wave_tile(
 	// flight (read kD steps after they are issued)
 	int known_t = total_u, known_l = total_u;
 	if (tid == 0) {
-		known_t = nb_top >= 0 ? wait_progress(top_prog, need_top(kD - 1)) : total_u;
-		known_l = nb_left >= 0 ? wait_progress(left_prog, need_left(kD - 1)) : total_u;
+		known_t = nb_top >= 0 ? wait_progress(top_prog, need_top(kD - 1), abortw, limit) : total_u;
+		known_l =
+			nb_left >= 0 ? wait_progress(left_prog, need_left(kD - 1), abortw, limit) : total_u;
 	}
 	__syncthreads();
 	T qe[kD], qh0[kD], qh1[kD];
@@ -338,8 +361,10 @@ __device__ __forceinline__ void wave_tile(
 			known_t = max(known_t, t_q);
 			known_l = max(known_l, l_q);
 			if (u2 < total_u) {
-				if (known_t < need_top(u2)) known_t = wait_progress(top_prog, need_top(u2));
-				if (known_l < need_left(u2)) known_l = wait_progress(left_prog, need_left(u2));
+				if (known_t < need_top(u2))
+					known_t = wait_progress(top_prog, need_top(u2), abortw, limit);
+				if (known_l < need_left(u2))
+					known_l = wait_progress(left_prog, need_left(u2), abortw, limit);
 			}
 			t_q = nb_top >= 0 ? coherent_load(top_prog) : total_u;
 			l_q = nb_left >= 0 ? coherent_load(left_prog) : total_u;
@@ -378,8 +403,10 @@ __global__ __launch_bounds__(kThreads) void ar_wave_kernel(
 	int* __restrict__ progress, const T* __restrict__ zero, T* __restrict__ sink,
 	T* __restrict__ mbox, int NB, int total_steps) {
 	extern __shared__ __align__(16) unsigned char smem_raw[];
+	// unbounded waits: tiles wait only on tiles with smaller workgroup ids,
+	// which were dispatched before them
 	wave_tile<T, TX, P0, KAHAN>(zeta, nt, nx, ny, phi0, phic, toff, P1, P2, nY, progress, zero, sink,
-	                            mbox, NB, total_steps, 0, blockIdx.x, smem_raw);
+	                            mbox, NB, total_steps, 0, blockIdx.x, nullptr, 0, smem_raw);
 }
 
 // ---- many surfaces in one persistent launch ----
@@ -419,18 +446,23 @@ int batch_bucket(int P0) {
 }
 
 // Workgroup w takes queue entries w, w + G, w + 2G, ... in order (G
-// workgroups, all resident).  The earliest unfinished entry then always
-// belongs to a workgroup that has finished its earlier entries and is
+// workgroups).  When all G are resident, the earliest unfinished entry
+// always belongs to a workgroup that has finished its earlier entries and is
 // marching that one, and it only waits on earlier entries, which are done:
-// the launch cannot deadlock.  The loop control is uniform (no atomic
-// fetch under a thread-0 branch), which keeps the compiled loop a plain
-// scalar loop around the tile's barriers.
+// the launch cannot deadlock.  Residency of the whole grid is the
+// precondition: the host never lets two batch launches run at once on a
+// device (they could each hold part of the CUs and wait on each other's
+// unscheduled workgroups), and the waits are bounded (abortw, limit) so that
+// a launch that still lost residency ends with an error instead of hanging.
+// The loop control is uniform (no atomic fetch under a thread-0 branch),
+// which keeps the compiled loop a plain scalar loop around the tile's
+// barriers.
 template <class T, int TX, bool KAHAN>
 __global__ __launch_bounds__(kThreads) void ar_batch_kernel(
 	T* __restrict__ arena, const T* __restrict__ coef, const int* __restrict__ taps,
 	int* __restrict__ progress, const T* __restrict__ zero, T* __restrict__ sink,
 	T* __restrict__ mbox, const BatchSurface* __restrict__ surf, const int2* __restrict__ queue,
-	int nqueue) {
+	int nqueue, int* __restrict__ abortw, long long limit) {
 	extern __shared__ __align__(16) unsigned char smem_raw[];
 	for (int item = blockIdx.x; item < nqueue; item += gridDim.x) {
 		const int si = queue[item].x;
@@ -447,7 +479,7 @@ __global__ __launch_bounds__(kThreads) void ar_batch_kernel(
 	case I:                                                                                   \
 		wave_tile<T, TX, kBatchP0[I], KAHAN>(z, sf.nt, sf.nx, sf.ny, c0, cc, tp, sf.P1, sf.P2,  \
 		                                     sf.nY, pr, zero, sink, mb, sf.NB, sf.total_steps, \
-		                                     sf.t0, b, smem_raw);                              \
+		                                     sf.t0, b, abortw, limit, smem_raw);               \
 		break;
 			ARMA_BATCH_BODY(0)
 			ARMA_BATCH_BODY(1)
@@ -576,17 +608,43 @@ const void* batch_kernel_fn(int precision, int tx) {
 
 }  // namespace
 
+namespace {
+
+// A batch launch is complete when its event has fired.  Events are shared so
+// that the chain below can outlive the batch that recorded them.
+struct LaunchEvent {
+	hipEvent_t ev = nullptr;
+	LaunchEvent() { ARMA_HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming)); }
+	~LaunchEvent() {
+		if (ev) (void)hipEventDestroy(ev);
+	}
+	LaunchEvent(const LaunchEvent&) = delete;
+	LaunchEvent& operator=(const LaunchEvent&) = delete;
+};
+
+// The last batch launch enqueued on each device, on any stream.
+std::mutex g_batch_mu;
+std::map<int, std::shared_ptr<LaunchEvent>> g_batch_last;
+
+// Bounded waits give up after this long without progress.
+constexpr double kBatchWaitLimitS = 10.0;
+
+}  // namespace
+
 struct ArBatch {
 	int precision = 0;
 	BatchArConfig cfg;
 	int nqueue = 0, grid = 0, max_slots = 0;
 	size_t lds = 0;
-	size_t nclear = 0;  // ints cleared before a launch: queue counter, progress, zero word
+	size_t nclear = 0;  // ints cleared before a launch: queue counter, abort word, progress, zero word
 	Scratch coef, taps, surf, queue, flags, mbox;
 	int* clear_base = nullptr;  // start of the words cleared before a launch
+	int* abortw = nullptr;      // set by a launch whose bounded waits gave up
 	int* progress = nullptr;
 	void* zero = nullptr;
 	void* sink = nullptr;
+	long long limit = 0;  // wall-clock ticks of kBatchWaitLimitS
+	std::shared_ptr<LaunchEvent> done;
 };
 
 namespace {
@@ -676,13 +734,15 @@ void build_batch(arma_hip_ctx* ctx, ArBatch* ab, const std::vector<BatchMember>&
 	upload(ctx, ab->taps.ptr, taps.data(), taps.size() * sizeof(int));
 	upload(ctx, ab->surf.ptr, surf.data(), surf.size() * sizeof(BatchSurface));
 	upload(ctx, ab->queue.ptr, queue.data(), queue.size() * sizeof(int2));
-	// flags: the queue counter (own line), progress counters, a zero word,
-	// and a sink for out-of-range stores sized for the largest surface
+	// flags: the queue counter and the abort word (one line each), progress
+	// counters, a zero word, and a sink for out-of-range stores sized for the
+	// largest surface
 	const size_t head = 256, prog_bytes = sizeof(int) * kProgStride * size_t(tiles);
 	const size_t zero_bytes = 256;
 	unsigned char* fb = (unsigned char*)ab->flags.get(head + prog_bytes + zero_bytes +
 	                                                  size_t(max_tiles) * kThreads * sizeof(T));
 	ab->clear_base = (int*)fb;
+	ab->abortw = (int*)(fb + 128);
 	ab->progress = (int*)(fb + head);
 	ab->zero = fb + head + prog_bytes;
 	ab->sink = fb + head + prog_bytes + zero_bytes;
@@ -698,6 +758,10 @@ void build_batch(arma_hip_ctx* ctx, ArBatch* ab, const std::vector<BatchMember>&
 	const int slots = ab->cfg.slots_per_cu > 0 ? std::min(ab->cfg.slots_per_cu, ab->max_slots)
 	                                           : ab->max_slots;
 	ab->grid = int(std::min<long long>(ab->nqueue, (long long)slots * std::max(ctx->num_cus, 1)));
+	int khz = 0;
+	ARMA_HIP_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device));
+	ab->limit = (long long)(double(std::max(khz, 1)) * 1e3 * kBatchWaitLimitS);
+	if (!ab->done) ab->done = std::make_shared<LaunchEvent>();
 }
 
 template <class T, int TX, bool KAHAN>
@@ -705,7 +769,25 @@ void batch_go(arma_hip_ctx* ctx, ArBatch* ab, T* arena) {
 	ar_batch_kernel<T, TX, KAHAN><<<ab->grid, kThreads, ab->lds, ctx->stream>>>(
 		arena, (const T*)ab->coef.ptr, (const int*)ab->taps.ptr, ab->progress,
 		(const T*)ab->zero, (T*)ab->sink, (T*)ab->mbox.ptr, (const BatchSurface*)ab->surf.ptr,
-		(const int2*)ab->queue.ptr, ab->nqueue);
+		(const int2*)ab->queue.ptr, ab->nqueue, ab->abortw, ab->limit);
+}
+
+void batch_enqueue(arma_hip_ctx* ctx, ArBatch* ab, void* arena) {
+	const int nclear = int(ab->nclear);
+	reset_progress<<<div_ceil(nclear, 256), 256, 0, ctx->stream>>>(ab->clear_base, nclear);
+	ARMA_HIP_CHECK(hipGetLastError());
+	KernelTimer timer(ctx, "ar_batch");
+	const int tx = ab->cfg.tile_x;
+	if (ab->precision == ARMA_FP32) {
+		if (tx == 8) batch_go<float, 8, true>(ctx, ab, (float*)arena);
+		else if (tx == 32) batch_go<float, 32, true>(ctx, ab, (float*)arena);
+		else batch_go<float, 16, true>(ctx, ab, (float*)arena);
+	} else {
+		if (tx == 8) batch_go<double, 8, false>(ctx, ab, (double*)arena);
+		else if (tx == 32) batch_go<double, 32, false>(ctx, ab, (double*)arena);
+		else batch_go<double, 16, false>(ctx, ab, (double*)arena);
+	}
+	ARMA_HIP_CHECK(hipGetLastError());
 }
 
 }  // namespace
@@ -764,22 +846,34 @@ size_t ar_batch_member_bytes(const arma_ar_desc* d, int tile_x) {
 	return es * (tiles * total_u * NB + tiles * kThreads) + sizeof(int) * kProgStride * tiles;
 }
 
+// Batch launches on one device run one after another, whatever stream they
+// are enqueued on: each waits for the previous one's event.  Under graph
+// capture the chain is left out (an event recorded outside the capture cannot
+// be waited on inside it); the caller then keeps replays from overlapping
+// other batch launches.
 void ar_batch_launch(arma_hip_ctx* ctx, ArBatch* ab, void* arena) {
-	const int nclear = int(ab->nclear);
-	reset_progress<<<div_ceil(nclear, 256), 256, 0, ctx->stream>>>(ab->clear_base, nclear);
-	ARMA_HIP_CHECK(hipGetLastError());
-	KernelTimer timer(ctx, "ar_batch");
-	const int tx = ab->cfg.tile_x;
-	if (ab->precision == ARMA_FP32) {
-		if (tx == 8) batch_go<float, 8, true>(ctx, ab, (float*)arena);
-		else if (tx == 32) batch_go<float, 32, true>(ctx, ab, (float*)arena);
-		else batch_go<float, 16, true>(ctx, ab, (float*)arena);
-	} else {
-		if (tx == 8) batch_go<double, 8, false>(ctx, ab, (double*)arena);
-		else if (tx == 32) batch_go<double, 32, false>(ctx, ab, (double*)arena);
-		else batch_go<double, 16, false>(ctx, ab, (double*)arena);
+	hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+	ARMA_HIP_CHECK(hipStreamIsCapturing(ctx->stream, &cap));
+	if (cap != hipStreamCaptureStatusNone) {
+		batch_enqueue(ctx, ab, arena);
+		return;
 	}
-	ARMA_HIP_CHECK(hipGetLastError());
+	std::lock_guard<std::mutex> lock(g_batch_mu);
+	std::shared_ptr<LaunchEvent>& last = g_batch_last[ctx->device];
+	if (last && last != ab->done) ARMA_HIP_CHECK(hipStreamWaitEvent(ctx->stream, last->ev, 0));
+	batch_enqueue(ctx, ab, arena);
+	ARMA_HIP_CHECK(hipEventRecord(ab->done->ev, ctx->stream));
+	last = ab->done;
+}
+
+void ar_batch_check(arma_hip_ctx* ctx, ArBatch* ab) {
+	int stalled = 0;
+	ARMA_HIP_CHECK(
+		hipMemcpyAsync(&stalled, ab->abortw, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+	ARMA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+	ARMA_HIP_REQUIRE(stalled == 0, ARMA_HIP_ERUNTIME,
+	                 "batch recurrence launch stalled (its workgroups were not all resident); "
+	                 "its surfaces are invalid");
 }
 
 void ar_batch_destroy(ArBatch* ab) { delete ab; }

@@ -203,7 +203,13 @@ void noise_batch_launch(arma_hip_ctx* ctx, NoiseBatch* b, void* arena);
 void noise_batch_destroy(NoiseBatch* b);
 ArBatch* ar_batch_create(arma_hip_ctx* ctx, int precision, const std::vector<BatchMember>& m,
                          const BatchArConfig& cfg);
+// Enqueues the persistent recurrence launch.  Launches on one device are
+// chained (each waits for the previous one, whatever its stream) outside
+// graph capture.
 void ar_batch_launch(arma_hip_ctx* ctx, ArBatch* b, void* arena);
+// Waits for the stream and throws ARMA_HIP_ERUNTIME when the last launch's
+// bounded waits gave up.
+void ar_batch_check(arma_hip_ctx* ctx, ArBatch* b);
 void ar_batch_destroy(ArBatch* b);
 int ar_batch_grid(const ArBatch* b);
 bool ar_batch_fits(const std::vector<BatchMember>& m, int tile_x);

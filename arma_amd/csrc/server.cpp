@@ -187,6 +187,7 @@ void run_job(arma_hip_server* s, Lane* L) {
 		}
 	}
 	ARMA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+	ar_batch_check(ctx, L->ar);
 }
 
 // Frees a request's device memory on the stream of the lane that made it

@@ -323,7 +323,17 @@ void arma_hip_partition_shape(const int32_t partition[3], const int32_t grid[3],
  * semantics of arma_hip_ar_generate followed by arma_hip_rescale on its own
  * descriptor, and its values do not depend on the rest of the batch, the
  * tile shape or the schedule.  A batch owns its device tables and scratch;
- * generate, capture and replay only enqueue kernels on the context stream. */
+ * generate, capture and replay only enqueue kernels on the context stream.
+ *
+ * The recurrence launch is persistent: its workgroups wait on each other, so
+ * all of them must be resident together.  Recurrence launches on one device
+ * therefore run one at a time, whatever context or stream enqueues them
+ * (each waits for the previous one); noise, rescale and other work still
+ * overlap.  A captured graph is outside that chain: do not replay it while
+ * another batch's recurrence runs on the device.  The waits inside a launch
+ * are bounded (10 s without progress); a launch that gave up makes the next
+ * call that waits for it (arma_hip_batch_factors, arma_hip_batch_tune, the
+ * server's lanes) return ARMA_HIP_ERUNTIME. */
 
 enum arma_batch_schedule {
 	ARMA_BATCH_AUTO = 0,
@@ -381,7 +391,8 @@ int arma_hip_batch_get_config(const arma_hip_batch* batch, arma_batch_config* co
                               int32_t* grid);
 int arma_hip_batch_set_config(arma_hip_ctx* ctx, arma_hip_batch* batch,
                               const arma_batch_config* config);
-/* Post-rescale factors of the last generate or replay (0 where none). */
+/* Post-rescale factors of the last generate or replay (0 where none); waits
+   for the context stream and reports a stalled recurrence launch. */
 int arma_hip_batch_factors(arma_hip_ctx* ctx, arma_hip_batch* batch, double* out);
 
 /* ---- Serving: continuous batching of surface and potential requests ----

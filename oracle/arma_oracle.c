@@ -28,8 +28,15 @@
  *                       brute force (GSL itself is absent: parity against GSL
  *                       bit patterns unpinned, tolerance 1e-15 relative).
  *   orc_derivative_t    pinned: exact on quadratics in t.
- *   orc_factor_waves    pinned: brute-force Python restatement of the same
- *                       scan and a closed form for a sinusoid.
+ *   orc_factor_waves    pinned: hand-worked known answers on built lines
+ *                       (event detection, parabola vertex, run
+ *                       representative = first largest |elevation|, the
+ *                       last run unscored, runs continuing across columns,
+ *                       no events), plus a run-by-run Python restatement on
+ *                       random slices.
+ *   orc_interp_edges    pinned: hand-worked values on a non-planar integer
+ *                       field (exact), including res(0,1) reading the fixed
+ *                       res(1,0) and res(2,0); misread orders fail them.
  *   orc_fft*            pinned: numpy.fft golden vectors.
  *   orc_window          pinned: closed form cosh(l(z+h))/cosh(lh) and the
  *                       rotate index map.

@@ -46,7 +46,11 @@ def test_potential_fp64_matches_oracle(hip_ctx, shape, npts, lb, ub):
     got, wn = run_gpu(hip_ctx, zeta, 12.0, lb, ub, npts, want_wn=True)
     ref, wref = O.linear_potential(zeta, (TWO_PI,) * 3, 12.0, lb, ub, npts, return_wnmax=True)
     np.testing.assert_allclose(wn, wref, rtol=1e-15)
-    assert np.max(np.abs(got - ref)) <= 1e-10 * np.max(np.abs(ref))
+    scale = np.max(np.abs(ref))
+    assert np.max(np.abs(got - ref)) <= 1e-10 * scale
+    # the fixed-up edges (row 0, column 0) on their own
+    assert np.max(np.abs(got[:, :, 0, :] - ref[:, :, 0, :])) <= 1e-10 * scale
+    assert np.max(np.abs(got[:, :, :, 0] - ref[:, :, :, 0])) <= 1e-10 * scale
 
 
 @pytest.mark.parametrize("shape,nz", [
@@ -148,7 +152,11 @@ def test_high_amplitude_fp64_matches_oracle(hip_ctx, shape, npts, lb, ub, delta)
                       delta=delta)
     ref, wref = O.high_amplitude_potential(zeta, delta, 12.0, lb, ub, npts, return_wnmax=True)
     np.testing.assert_allclose(wn, wref, rtol=1e-15)
-    assert np.max(np.abs(got - ref)) <= 1e-10 * np.max(np.abs(ref))
+    scale = np.max(np.abs(ref))
+    assert np.max(np.abs(got - ref)) <= 1e-10 * scale
+    # the fixed-up edges (row 0, column 0) on their own
+    assert np.max(np.abs(got[:, :, 0, :] - ref[:, :, 0, :])) <= 1e-10 * scale
+    assert np.max(np.abs(got[:, :, :, 0] - ref[:, :, :, 0])) <= 1e-10 * scale
 
 
 def test_high_amplitude_fp32(hip_ctx):

@@ -206,3 +206,39 @@ def test_server_stream_rules(hip_ctx, setup):
             srv.submit_surface(d, 1.0, pot_desc(hip_ctx, (20, 20, 33), 1))
     finally:
         srv.close()
+
+
+def test_server_lanes_beyond_residency(hip_ctx, setup):
+    """Lanes whose batches each hold more tiles than one recurrence launch
+    keeps resident: the launches run one after another on the device (two
+    partly resident persistent launches could wait on each other's
+    unscheduled workgroups), and every result equals the surface made alone."""
+    params, models = setup
+    shape, part = (24, 640, 640), (24, 160, 160)
+    descs = [surface_desc(hip_ctx, params, models, shape, 1, part, 50 + i) for i in range(4)]
+    gamma0 = float(models[1][0][0, 0, 0])
+    probe = arma_amd.Batch(hip_ctx, descs[:1], [gamma0], PREC, (16, 1, 0))
+    try:
+        _, grid = probe.config()
+    finally:
+        probe.close()
+    tiles = (640 // 16) ** 2
+    print(f"tiles per batch {tiles}, resident workgroups per launch {grid}", flush=True)
+    assert grid < tiles
+    srv = arma_amd.Server(0, lanes=3, budget=8 << 30, precision=PREC, max_batch=1)
+    try:
+        ids = {srv.submit_surface(d, gamma0): i for i, d in enumerate(descs)}
+        got = {}
+        while len(got) < len(ids):
+            for c in srv.poll(8, wait_ms=60000):
+                assert c.status == 0, c.status
+                assert c.batch_size == 1
+                got[ids[c.id]] = srv.copy_out(c.zeta, tuple(c.shape))
+                srv.release(c.id)
+    finally:
+        srv.close()
+    for i, d in enumerate(descs):
+        z = torch.empty(shape, dtype=torch.float32, device=DEV)
+        hip_ctx.ar_generate(d, z)
+        hip_ctx.rescale(z, shape, PREC, gamma0)
+        np.testing.assert_array_equal(got[i], z.cpu().numpy(), err_msg=f"surface {i}")

@@ -4,8 +4,10 @@ The reference's Linear_solver (src/velocity/linear_solver.cc:20-184) has no
 tests of its own; these checks pin the transform convention (numpy golden
 vectors), the window (closed form cosh(l(z+h))/cosh(lh) and the rotate map
 of src/blitz.hh:54-71), the derivative (exact on quadratics), factor_waves
-(an independent restatement of src/factor_waves.cc:8-89) and the whole
-solver (a single Fourier mode has an analytic answer).
+(a run-by-run restatement of src/factor_waves.cc:8-89 on random slices; the
+hand-worked known answers are in test_oracle_pins.py) and the whole solver
+(a single Fourier mode has an analytic answer; numpy restatements with the
+edge fix-up in closed form).
 """
 import os
 
@@ -116,6 +118,18 @@ def test_factor_waves_restatement(seed):
     np.testing.assert_allclose(got, (kx[0], ky[0], kx[1], ky[1]), rtol=1e-13)
 
 
+def edges_numpy(res):
+    """Edge fix-up of src/velocity/basic_solver.cc:61-68 in the closed forms
+    that tests/test_oracle_pins.py checks against hand-worked values."""
+    r = res.copy()
+    r[0, 0] = 3 * r[1, 1] - r[2, 1] - r[1, 2]
+    for k in range(1, r.shape[0]):
+        r[k, 0] = r[k, 1] + r[k - 1, 1] - r[k - 1, 2]
+    for l in range(1, r.shape[1]):
+        r[0, l] = r[1, l] + r[1, l - 1] - r[2, l - 1]
+    return r
+
+
 def potential_numpy(zeta, delta, depth, lb, ub, npts):
     """numpy restatement of the linear solver (independent of the oracle)."""
     nt, nx, ny = zeta.shape
@@ -149,7 +163,7 @@ def potential_numpy(zeta, delta, depth, lb, ub, npts):
             w[~np.isfinite(w)] = 0
             w = rotate_ref(w)
             res = np.real(np.fft.ifft2(F * w))
-            out[i, j] = O.interp_edges(res)
+            out[i, j] = edges_numpy(res)
     return out
 
 
@@ -271,7 +285,7 @@ def ha_potential_numpy(zeta, delta, depth, lb, ub, npts, conj=False):
         G = np.fft.fft2(F)
         for j in range(npts[1]):
             w = O.window(nx, ny, 1 / kx, 1 / ky, lb[1] + dzd * j, depth)
-            out[i, j] = O.interp_edges(np.real(np.fft.ifft2(G * w)))
+            out[i, j] = edges_numpy(np.real(np.fft.ifft2(G * w)))
     return out
 
 
