@@ -45,7 +45,8 @@ EXPORTS = [
     "arma_hip_server_release", "arma_hip_server_stats", "arma_hip_server_copy",
     "arma_hip_yule_walker_choi_device", "arma_hip_white_noise_variance_device",
     "arma_hip_validate_process", "arma_hip_nit_realisation", "arma_hip_acf_generate", "arma_hip_writer_open", "arma_hip_writer_append",
-    "arma_hip_writer_close",
+    "arma_hip_writer_close", "arma_hip_acf_generate_device",
+    "arma_hip_yule_walker_choi_device_many",
 ]
 
 
@@ -93,9 +94,12 @@ WAVE_SIN, WAVE_COS, WAVE_STOKES, WAVE_STANDING = 0, 1, 2, 3
 
 
 def acf_generate(func=WAVE_COS, amplitude=1.0, velocity=1.0, beta=(0.8, 0.0),
-                 alpha=(0.06, 0.06, 0.06), nwaves=(1.5, 1.5, 1.5), shape=(16, 16, 16)):
+                 alpha=(0.06, 0.06, 0.06), nwaves=(1.5, 1.5, 1.5), shape=(16, 16, 16),
+                 ctx=None):
     """ACF_generator (src/generator/acf_generator.cc:130-215): (acf, delta).
-    Defaults are the reference's (src/generator/acf_generator.hh:25-34)."""
+    Defaults are the reference's (src/generator/acf_generator.hh:25-34).
+    With a Context the auto-covariance runs on its GPU
+    (arma_hip_acf_generate_device), else on the host."""
     d = AcfGenDesc()
     d.func, d.amplitude, d.velocity = func, amplitude, velocity
     d.beta[:] = list(beta)
@@ -106,7 +110,10 @@ def acf_generate(func=WAVE_COS, amplitude=1.0, velocity=1.0, beta=(0.8, 0.0),
     out = np.empty(cap, np.float64)
     so = (C.c_int32 * 3)()
     de = (C.c_double * 3)()
-    check(lib().arma_hip_acf_generate(C.byref(d), out, cap, so, de))
+    if ctx is None:
+        check(lib().arma_hip_acf_generate(C.byref(d), out, cap, so, de))
+    else:
+        check(lib().arma_hip_acf_generate_device(ctx.h, C.byref(d), out, cap, so, de))
     s = tuple(so)
     return out[:int(np.prod(s))].reshape(s).copy(), tuple(de)
 
@@ -243,6 +250,8 @@ def lib():
         L.arma_hip_validate_process.argtypes = [dp, i3, C.POINTER(C.c_int32), vp, C.c_int32]
         L.arma_hip_acf_generate.argtypes = [C.POINTER(AcfGenDesc), dp, C.c_size_t,
                                             C.POINTER(C.c_int32), C.POINTER(C.c_double)]
+        L.arma_hip_acf_generate_device.argtypes = [vp, C.POINTER(AcfGenDesc), dp, C.c_size_t,
+                                                   C.POINTER(C.c_int32), C.POINTER(C.c_double)]
         L.arma_hip_nit_realisation.argtypes = [vp, vp, i3, C.c_int32, C.c_double,
                                                 C.POINTER(NitDesc)]
         L.arma_hip_writer_open.argtypes = [vp, C.c_char_p, C.c_int32, C.c_size_t, C.POINTER(vp)]
@@ -279,6 +288,11 @@ def lib():
         L.arma_hip_yule_walker_choi_device.argtypes = [vp, dp, C.c_int32, C.c_int32, C.c_double,
                                                        C.c_int32, C.c_int32, C.c_double, dp,
                                                        i3, C.POINTER(C.c_double)]
+        i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+        i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+        L.arma_hip_yule_walker_choi_device_many.argtypes = [
+            vp, C.c_int32, dp, i64p, i32p, i32p, C.c_double, C.c_int32, C.c_int32, C.c_double,
+            dp, i64p, i32p, dp, i32p]
         L.arma_hip_white_noise_variance_device.argtypes = [vp, dp, i3, dp, i3,
                                                            C.POINTER(C.c_double)]
         L.arma_hip_profile.argtypes = [vp, C.c_int32]
@@ -585,6 +599,35 @@ class Context:
             1 if chop else 0, chop_eps, out, order, C.byref(var)))
         shape = tuple(order)
         return out[:int(np.prod(shape))].reshape(shape).copy(), var.value
+
+    def yule_walker_choi_many(self, acfs, max_orders, var_eps=1e-6, determine_order=True,
+                              chop=True, chop_eps=1e-10):
+        """Choi coefficients of several models in one GPU launch (one
+        workgroup per model): a list of (phi, solver variance), or None for a
+        model whose recursion failed."""
+        cubes = [np.ascontiguousarray(a, np.float64) for a in acfs]
+        ns = np.array([c.shape[0] for c in cubes], np.int32)
+        mo = np.array([int(m) for m in max_orders], np.int32)
+        acf_off = np.cumsum([0] + [c.size for c in cubes[:-1]]).astype(np.int64)
+        cap = [max(int(m), 2) ** 3 + int(n) ** 3 for m, n in zip(mo, ns)]
+        phi_off = np.cumsum([0] + cap[:-1]).astype(np.int64)
+        flat = np.concatenate([c.reshape(-1) for c in cubes]) if cubes else np.zeros(1)
+        out = np.zeros(max(1, sum(cap)), np.float64)
+        order = np.zeros(3 * len(cubes), np.int32)
+        var = np.zeros(len(cubes), np.float64)
+        status = np.zeros(len(cubes), np.int32)
+        check(lib().arma_hip_yule_walker_choi_device_many(
+            self.h, len(cubes), flat, acf_off, ns, mo, var_eps, 1 if determine_order else 0,
+            1 if chop else 0, chop_eps, out, phi_off, order, var, status))
+        res = []
+        for i in range(len(cubes)):
+            if status[i] != 0:
+                res.append(None)
+                continue
+            shp = tuple(int(v) for v in order[3 * i:3 * i + 3])
+            o = int(phi_off[i])
+            res.append((out[o:o + int(np.prod(shp))].reshape(shp).copy(), float(var[i])))
+        return res
 
     def white_noise_variance(self, acf_cube, phi):
         """AR white-noise variance in FP64 on this context's GPU."""

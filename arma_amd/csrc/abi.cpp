@@ -168,6 +168,12 @@ void arma_hip_destroy(arma_hip_ctx* ctx) {
 	                             &ctx->zeta_tmp, &ctx->phi_tmp, &ctx->writer_stage})
 		s->release();
 	if (ctx->writer_host) (void)hipHostFree(ctx->writer_host);
+	if (ctx->aux_stream) {
+		(void)hipStreamSynchronize(ctx->aux_stream);
+		(void)hipStreamDestroy(ctx->aux_stream);
+	}
+	for (hipEvent_t e : ctx->pot_events)
+		if (e) (void)hipEventDestroy(e);
 	if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
 	delete ctx;
 }
@@ -702,6 +708,47 @@ int arma_hip_yule_walker_choi_device(arma_hip_ctx* ctx, const double* acf, int32
 					phi_out[(size_t(a) * shp[1] + b) * shp[2] + c] = phi[(size_t(a) * l + b) * l + c];
 		for (int k = 0; k < 3; ++k) order_out[k] = shp[k];
 		if (varwn_solver) *varwn_solver = var;
+	});
+}
+
+int arma_hip_yule_walker_choi_device_many(arma_hip_ctx* ctx, int32_t count, const double* acf,
+                                          const int64_t* acf_off, const int32_t* n,
+                                          const int32_t* max_order, double var_eps,
+                                          int32_t determine_order, int32_t do_chop,
+                                          double chop_eps, double* phi_out,
+                                          const int64_t* phi_off, int32_t* order_out,
+                                          double* varwn_solver, int32_t* status) {
+	return guarded([&] {
+		use_device(ctx);
+		ARMA_HIP_REQUIRE(count >= 0, ARMA_HIP_EINVAL, "bad model count");
+		if (count == 0) return;
+		ARMA_HIP_REQUIRE(acf && acf_off && n && max_order && phi_out && phi_off && order_out &&
+		                     status,
+		                 ARMA_HIP_EINVAL, "null argument");
+		std::vector<arma_hip::YwProblem> probs(count);
+		for (int i = 0; i < count; ++i) {
+			ARMA_HIP_REQUIRE(n[i] >= 3, ARMA_HIP_EINVAL, "bad Yule-Walker argument");
+			probs[i] = arma_hip::YwProblem{acf + acf_off[i], n[i], max_order[i], var_eps,
+			                               determine_order != 0};
+		}
+		std::vector<arma_hip::YwSolution> sols;
+		arma_hip::yule_walker_choi_device_many(ctx, probs, sols);
+		for (int i = 0; i < count; ++i) {
+			const arma_hip::YwSolution& r = sols[i];
+			status[i] = r.status;
+			if (r.status != ARMA_HIP_OK) continue;
+			const int l = r.l;
+			int shp[3] = {l, l, l};
+			if (do_chop) arma_hip::chop_shape(r.phi, shp, chop_eps, shp);
+			double* out = phi_out + phi_off[i];
+			for (int a = 0; a < shp[0]; ++a)
+				for (int b = 0; b < shp[1]; ++b)
+					for (int c = 0; c < shp[2]; ++c)
+						out[(size_t(a) * shp[1] + b) * shp[2] + c] =
+							r.phi[(size_t(a) * l + b) * l + c];
+			for (int k = 0; k < 3; ++k) order_out[3 * i + k] = shp[k];
+			if (varwn_solver) varwn_solver[i] = r.varwn;
+		}
 	});
 }
 

@@ -533,7 +533,8 @@ void launch_typed(arma_hip_ctx* ctx, const arma_ar_desc* d, T* zeta) {
 	std::vector<int> offs;
 	const size_t n0 = append_tables(d->phi, P0, P1, P2, P0, kTY + P2 - 1, ph, offs);
 	const size_t phi_bytes = (ph.size() * sizeof(T) + 255) / 256 * 256;
-	unsigned char* dbuf = (unsigned char*)ctx->phi.get(phi_bytes + offs.size() * sizeof(int) + 4);
+	unsigned char* dbuf =
+		(unsigned char*)ctx->phi.get(phi_bytes + offs.size() * sizeof(int) + 4);
 	T* dphi = (T*)dbuf;
 	int* doffs = (int*)(dbuf + phi_bytes);
 	upload(ctx, dphi, ph.data(), ph.size() * sizeof(T));
@@ -545,8 +546,7 @@ void launch_typed(arma_hip_ctx* ctx, const arma_ar_desc* d, T* zeta) {
 	// and a per-tile sink for out-of-range stores
 	const size_t prog_bytes = sizeof(int) * kProgStride * size_t(ntiles);
 	const size_t zero_bytes = 256;
-	unsigned char* fb = (unsigned char*)ctx->flags.get(prog_bytes + zero_bytes +
-	                                                   size_t(ntiles) * kThreads * sizeof(T));
+	unsigned char* fb = (unsigned char*)ctx->flags.get(prog_bytes + zero_bytes + size_t(ntiles) * kThreads * sizeof(T));
 	int* prog = (int*)fb;
 	const T* zero = (const T*)(fb + prog_bytes);
 	T* sink = (T*)(fb + prog_bytes + zero_bytes);

@@ -93,6 +93,11 @@ struct arma_hip_ctx {
 	    zeta_tmp, phi_tmp;
 	arma_hip::Scratch yw;
 	int twiddle_n = 0;
+	// a second stream (default priority) on which the potential's row pass of
+	// one chunk overlaps the column pass of the next, and the events that
+	// order the two over their ping-pong intermediate buffers
+	hipStream_t aux_stream = nullptr;
+	hipEvent_t pot_events[4] = {};
 	// batch configurations chosen by the autotuner, by batch signature:
 	// tile width, schedule, workgroups per CU
 	std::map<std::string, std::vector<int>> batch_tuned;
@@ -117,17 +122,19 @@ inline int div_ceil(int a, int b) { return a / b + (a % b ? 1 : 0); }
 // context has profiling enabled.
 struct KernelTimer {
 	arma_hip_ctx* ctx;
+	hipStream_t stream;
 	ProfileRecord rec{};
-	KernelTimer(arma_hip_ctx* c, const char* name) : ctx(c) {
+	KernelTimer(arma_hip_ctx* c, const char* name) : KernelTimer(c, name, c->stream) {}
+	KernelTimer(arma_hip_ctx* c, const char* name, hipStream_t s) : ctx(c), stream(s) {
 		if (!ctx->profiling) return;
 		rec.name = name;
 		ARMA_HIP_CHECK(hipEventCreate(&rec.start));
 		ARMA_HIP_CHECK(hipEventCreate(&rec.stop));
-		ARMA_HIP_CHECK(hipEventRecord(rec.start, ctx->stream));
+		ARMA_HIP_CHECK(hipEventRecord(rec.start, stream));
 	}
 	~KernelTimer() {
 		if (!ctx->profiling) return;
-		if (hipEventRecord(rec.stop, ctx->stream) == hipSuccess) ctx->pending.push_back(rec);
+		if (hipEventRecord(rec.stop, stream) == hipSuccess) ctx->pending.push_back(rec);
 	}
 };
 
@@ -155,6 +162,21 @@ double launch_rescale(arma_hip_ctx* ctx, void* zeta, const int32_t shape[3], int
 void yule_walker_choi_device(arma_hip_ctx* ctx, const double* acf, int n, int max_order,
                              double var_eps, bool determine, std::vector<double>& phi, int& l,
                              double& varwn);
+// Several Choi recursions in one launch (one workgroup per model).
+struct YwProblem {
+	const double* acf;  // n^3 cube
+	int n, max_order;
+	double var_eps;
+	bool determine;
+};
+struct YwSolution {
+	int status = ARMA_HIP_ERUNTIME;
+	int l = 0;
+	double varwn = 0.0;
+	std::vector<double> phi;  // l^3, before chopping
+};
+void yule_walker_choi_device_many(arma_hip_ctx* ctx, const std::vector<YwProblem>& probs,
+                                  std::vector<YwSolution>& sols);
 double white_noise_variance_device(arma_hip_ctx* ctx, const double* acf, const int32_t acf_shape[3],
                                    const double* phi, const int32_t phi_shape[3]);
 // Roots of c[0] + c[1] x + ... + c[n-1] x^(n-1) (eigenvalues of the balanced
@@ -162,6 +184,12 @@ double white_noise_variance_device(arma_hip_ctx* ctx, const double* acf, const i
 // leading coefficient is zero or the iteration does not converge.
 int polynomial_roots(const std::vector<double>& c, std::vector<double>& re,
                      std::vector<double>& im);
+// ACF_generator up to the auto-covariance (the decayed, rescaled plain wave,
+// extents s) and the crop of the auto-covariance to its first half (h, and
+// the ACF's shape and grid spacing).
+void acf_gen_wave(const arma_acf_gen_desc& d, std::vector<double>& we, int s[3]);
+void acf_gen_crop(const arma_acf_gen_desc& d, const int s[3], int h[3], int32_t shape_out[3],
+                  double delta_out[3]);
 // chop (src/chop.cc:4-26) of an n0 x n1 x n2 array: the trimmed shape.
 void chop_shape(const std::vector<double>& phi, const int shape[3], double eps, int out[3]);
 

@@ -563,6 +563,162 @@ __global__ __launch_bounds__(64) void noise_seg_kernel(
 	}
 }
 
+// ---- one stream per wave, the MT521 state in registers ----
+//
+// For the dynamic-creator records of p = 521 with 32-bit words (nn = 17,
+// mm = 8, the records the partitioned configurations use), a wave runs one
+// stream with state word k in lane k.  A twist, in genrand_mt's sequential
+// order (src/parallel_mt.hh:236), is
+//   new[k]  = tw(old[k], old[k+1], old[k+8])     k = 0..8
+//   new[k]  = tw(old[k], old[k+1], new[k-9])     k = 9..15
+//   new[16] = tw(old[16], new[0], new[7])
+// which takes three row shifts inside the wave's first 16-lane row (DPP)
+// and three lane reads for the values that cross into or out of lane 16.
+// Each round tempers twists into the wave's LDS word buffer until 64
+// attempts' worth is there, evaluates the 64 polar attempts in parallel and
+// places the accepted pairs where the sequential generator puts them.  The
+// four streams of a workgroup are independent waves: their LDS accesses
+// are ordered within the wave, with no workgroup barrier.
+
+constexpr int kWaveNN = 17, kWaveMM = 8;
+
+__device__ __forceinline__ uint32_t row_shl1(uint32_t v) {
+	return uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x101, 0xf, 0xf, true));
+}
+__device__ __forceinline__ uint32_t row_shl8(uint32_t v) {
+	return uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x108, 0xf, 0xf, true));
+}
+__device__ __forceinline__ uint32_t row_shr9(uint32_t v) {
+	return uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x119, 0xf, 0xf, true));
+}
+
+__device__ __forceinline__ void wave_lds_order() {
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <class OutT, int SEM, class Geo>
+__global__ __launch_bounds__(256) void noise_wave_kernel(const arma_mt_params* __restrict__ params,
+                                                         const uint32_t* __restrict__ seeds,
+                                                         Geo geo, int nstreams) {
+	constexpr int WPA = SEM == ARMA_NOISE_DOUBLE ? 4 : 2;  // words per attempt
+	constexpr int RW = 64 * WPA;                            // words per round
+	__shared__ uint32_t buf_all[4][RW + 32];
+	const int lane = threadIdx.x & 63;
+	const int pid = __builtin_amdgcn_readfirstlane(int(blockIdx.x) * 4 + int(threadIdx.x >> 6));
+	const bool valid = pid < nstreams;
+	uint32_t* buf = buf_all[threadIdx.x >> 6];
+	const arma_mt_params p = params[valid ? pid : 0];
+	const StreamBlock<OutT> blk = geo(valid ? pid : 0);
+	OutT* const out = blk.out;
+	const int sy = blk.sy, sz = blk.sz;
+	const int lo0 = blk.lo0, lo1 = blk.lo1, lo2 = blk.lo2;
+	const int e2 = blk.e2;
+	const double stddev = blk.sd;
+	const float stddev_f = blk.sdf;
+	const long long e12 = (long long)blk.e1 * e2;
+	const long long n = valid ? (long long)blk.e0 * e12 : 0;
+	const double r12 = 1.0 / double(e12), r2 = 1.0 / double(e2);
+	const bool pow2 = (e2 & (e2 - 1)) == 0 && (e12 & (e12 - 1)) == 0;
+	const int s2 = __ffs(unsigned(e2)) - 1, s12 = __ffsll((unsigned long long)e12) - 1;
+	auto put = [&](long long q, OutT v) {
+		// q < 2^31: 32-bit quotients
+		const uint32_t uq = uint32_t(q), u12 = uint32_t(e12), u2 = uint32_t(e2);
+		uint32_t qt, rr, qx;
+		if (pow2) {
+			qt = uq >> s12;
+			rr = uq & (u12 - 1u);
+			qx = rr >> s2;
+		} else {
+			qt = uint32_t(double(uq) * r12);
+			if (uint64_t(qt) * u12 > uq) --qt;
+			else if (uint64_t(qt + 1) * u12 <= uq) ++qt;
+			rr = uq - qt * u12;
+			qx = uint32_t(double(rr) * r2);
+			if (uint64_t(qx) * u2 > rr) --qx;
+			else if (uint64_t(qx + 1) * u2 <= rr) ++qx;
+		}
+		out[(size_t(lo0 + qt) * sy + (lo1 + qx)) * sz + (lo2 + (rr - qx * u2))] = v;
+	};
+	// sgenrand_mt: word k of the recurrence from the seed (lane k keeps its own)
+	uint32_t s = 0;
+	{
+		uint32_t x = valid && seeds ? seeds[pid] : 0u;
+		for (int k = 0; k < kWaveNN; ++k) {
+			if (k == lane) s = x & p.wmask;
+			x = 1812433253u * (x ^ (x >> 30)) + uint32_t(k + 1);
+		}
+	}
+	int avail = 0;
+	long long produced = 0;
+	while (produced < n) {
+		while (avail < RW) {
+			// one twist of the 17 words (every lane takes part in the shifts)
+			const uint32_t s16 = uint32_t(__builtin_amdgcn_readlane(int(s), 16));
+			uint32_t nx1 = row_shl1(s), nx8 = row_shl8(s);
+			if (lane == 15) nx1 = s16;
+			if (lane == 8) nx8 = s16;
+			const uint32_t a = mt_twist_word(s, nx1, nx8, p);  // lanes 0..8
+			const uint32_t a9 = row_shr9(a);                   // lane k >= 9: a[k - 9]
+			const uint32_t n0 = uint32_t(__builtin_amdgcn_readlane(int(a), 0));
+			const uint32_t n7 = uint32_t(__builtin_amdgcn_readlane(int(a), kWaveMM - 1));
+			const uint32_t b = mt_twist_word(s, nx1, a9, p);  // lanes 9..15
+			const uint32_t c = mt_twist_word(s, n0, n7, p);   // lane 16
+			s = lane < kWaveNN - kWaveMM ? a : (lane < kWaveNN - 1 ? b : c);
+			if (lane < kWaveNN) buf[avail + lane] = mt_temper(s, p);
+			avail += kWaveNN;
+		}
+		wave_lds_order();
+		// one attempt per lane
+		const uint32_t* wv = buf + lane * WPA;
+		bool accept;
+		double xd = 0, yd = 0, r2d = 0;
+		float xf = 0, yf = 0, r2f = 0;
+		if (SEM == ARMA_NOISE_DOUBLE) {
+			xd = 2.0 * canonical_from(wv[0], wv[1]) - 1.0;
+			yd = 2.0 * canonical_from(wv[2], wv[3]) - 1.0;
+			r2d = xd * xd + yd * yd;
+			accept = !(r2d > 1.0 || r2d == 0.0);
+		} else {
+			xf = 2.0f * canonical_from_f(wv[0]) - 1.0f;
+			yf = 2.0f * canonical_from_f(wv[1]) - 1.0f;
+			r2f = xf * xf + yf * yf;
+			accept = !(r2f > 1.0f || r2f == 0.0f);
+		}
+		const unsigned long long mask = __ballot(accept);
+		const int below = __popcll(mask & ((1ull << lane) - 1ull));
+		const int total = __popcll(mask);
+		if (accept) {
+			const long long pos = produced + 2ll * below;
+			if (pos < n) {
+				if (SEM == ARMA_NOISE_DOUBLE) {
+					const double mult = sqrt(-2.0 * log(r2d) / r2d);
+					put(pos, OutT(yd * mult * stddev + 0.0));
+					if (pos + 1 < n) put(pos + 1, OutT(xd * mult * stddev + 0.0));
+				} else {
+					const float mult = sqrtf(-2.0f * logf(r2f) / r2f);
+					put(pos, OutT(yf * mult * stddev_f + 0.0f));
+					if (pos + 1 < n) put(pos + 1, OutT(xf * mult * stddev_f + 0.0f));
+				}
+			}
+		}
+		produced += 2ll * total;
+		// keep the words past the round (fewer than 17)
+		const int left = avail - RW;
+		const uint32_t keep = lane < left ? buf[RW + lane] : 0u;
+		wave_lds_order();
+		if (lane < left) buf[lane] = keep;
+		avail = left;
+	}
+}
+
+bool wave_kernel_fits(const arma_mt_params* mts, int n) {
+	for (int i = 0; i < n; ++i)
+		if (mts[i].nn != kWaveNN || mts[i].mm != kWaveMM) return false;
+	return true;
+}
+
 // The two-per-wave kernel applies to states of at most 32 words whose twist
 // needs only old words for its redundant evaluations.
 bool seg_kernel_fits(const arma_mt_params* mts, int n) {
@@ -603,6 +759,9 @@ void launch_streams(arma_hip_ctx* ctx, const arma_mt_params* hparams,
 		// the whole grid from one stream (framework=none)
 		noise_stream_kernel<OutT, SEM, 256, kMaxNN, Geo>
 			<<<1, 256, 0, ctx->stream>>>(dparams, dseeds, geo);
+	} else if (small_blocks && wave_kernel_fits(hparams, nstreams)) {
+		noise_wave_kernel<OutT, SEM, Geo>
+			<<<div_ceil(nstreams, 4), 256, 0, ctx->stream>>>(dparams, dseeds, geo, nstreams);
 	} else if (small_blocks && seg_kernel_fits(hparams, nstreams)) {
 		noise_seg_kernel<OutT, SEM, Geo>
 			<<<div_ceil(nstreams, 2), 64, 0, ctx->stream>>>(dparams, dseeds, geo, nstreams);

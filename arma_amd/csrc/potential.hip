@@ -664,11 +664,11 @@ struct WinParts {
 		}
 		const R u = R(du * px), v = R(dv * py);
 		l = R(kTwoPi) * sqrt(u * u + v * v);
-		scale = R(kFourPi) / (R(1) + exp(R(-2) * l * R(h)));
+		const R e2h = exp(R(-2) * l * R(h));
+		scale = R(kFourPi) / (R(1) + e2h);
 		direct = double(l) * fabs(dz) > kDirect;
-		c = exp(-l * R(zs + 2.0 * h));
 		rho = exp(l * R(dz));
-		irho = exp(-l * R(dz));
+		irho = R(1) / rho;
 		// first level where l z > kLow
 		int js = 0;
 		if (double(l) * zs < kLow && dz > 0.0) {
@@ -677,6 +677,11 @@ struct WinParts {
 		}
 		jstart = js;
 		a = js < nz ? exp(l * R(zs + dz * js)) : R(0);
+		// c_0 = e^{-l (zs + 2h)} = e^{-2lh} / e^{l zs} while both factors are
+		// in range (a few ulps from the direct exponential), else directly
+		const bool split = js == 0 && !direct && double(l) * fabs(zs) < kDirect &&
+		                   double(l) * 2.0 * h < kDirect;
+		c = split ? e2h / a : exp(-l * R(zs + 2.0 * h));
 	}
 	// W_j for j = 0, 1, 2, ... in order
 	__device__ __forceinline__ R next(int j, double zs, double dz, double h) {
@@ -1180,13 +1185,30 @@ void launch_inverse(arma_hip_ctx* ctx, const PotPlan& p, const double2* twx, con
 	const size_t lds_r = sizeof(cpx<T>) * size_t(xsize<RN1, RN2>(R));
 	allow_lds(&inv_cols_kernel<T, CN1, CN2>, lds_c);
 	allow_lds(&inv_rows_kernel<T, RN1, RN2>, lds_r);
-	// the intermediate of a whole chunk round-trips through HBM; chunks are
-	// large so that the launch count stays small
+	// the column-transformed intermediate of a chunk of output slices goes
+	// from the column pass to the row pass through device memory.  With more
+	// than one chunk, two intermediate buffers alternate and the row pass of
+	// chunk k (on the context's second stream) overlaps the column pass of
+	// chunk k + 1 (on its main stream)
 	const size_t per_slice = sizeof(cpx<T>) * size_t(p.nz) * M * NX;
 	const int chunk = std::min(p.nto, int(std::max<size_t>(1, chunk_bytes() / per_slice)));
-	cpx<T>* G = (cpx<T>*)ctx->pot_mid.get(per_slice * chunk);
-	for (int i0 = 0; i0 < p.nto; i0 += chunk) {
-		const int cn = std::min(chunk, p.nto - i0);
+	const int nchunks = div_ceil(p.nto, chunk);
+	const size_t gsize = per_slice * chunk;
+	unsigned char* gbase = (unsigned char*)ctx->pot_mid.get(gsize * (nchunks > 1 ? 2 : 1));
+	hipStream_t rows_stream = ctx->stream;
+	hipEvent_t* ev = ctx->pot_events;  // [0, 1]: columns done, [2, 3]: rows done
+	if (nchunks > 1) {
+		if (!ctx->aux_stream)
+			ARMA_HIP_CHECK(hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking));
+		for (int e = 0; e < 4; ++e)
+			if (!ev[e]) ARMA_HIP_CHECK(hipEventCreateWithFlags(&ev[e], hipEventDisableTiming));
+		rows_stream = ctx->aux_stream;
+	}
+	for (int k = 0; k < nchunks; ++k) {
+		const int i0 = k * chunk, cn = std::min(chunk, p.nto - i0), b = k & 1;
+		cpx<T>* G = (cpx<T>*)(gbase + gsize * (nchunks > 1 ? b : 0));
+		// the column pass reuses buffer b once the rows of chunk k - 2 are done
+		if (k >= 2) ARMA_HIP_CHECK(hipStreamWaitEvent(ctx->stream, ev[2 + b], 0));
 		{
 			KernelTimer t(ctx, "pot_inv_cols");
 			inv_cols_kernel<T, CN1, CN2><<<dim3(M / CB, cn), kThreads, lds_c, ctx->stream>>>(
@@ -1195,14 +1217,22 @@ void launch_inverse(arma_hip_ctx* ctx, const PotPlan& p, const double2* twx, con
 				G, CB, ZB);
 			ARMA_HIP_CHECK(hipGetLastError());
 		}
+		if (nchunks > 1) {
+			ARMA_HIP_CHECK(hipEventRecord(ev[b], ctx->stream));
+			ARMA_HIP_CHECK(hipStreamWaitEvent(rows_stream, ev[b], 0));
+		}
 		{
-			KernelTimer t(ctx, "pot_inv_rows");
+			KernelTimer t(ctx, "pot_inv_rows", rows_stream);
 			inv_rows_kernel<T, RN1, RN2><<<dim3(div_ceil(NX, R), cn * p.nz), kThreads, lds_r,
-			                               ctx->stream>>>(NX, 2 * M, p.nz, i0, twy, G, phi, edge,
+			                               rows_stream>>>(NX, 2 * M, p.nz, i0, twy, G, phi, edge,
 			                                              R);
 			ARMA_HIP_CHECK(hipGetLastError());
 		}
+		if (nchunks > 1) ARMA_HIP_CHECK(hipEventRecord(ev[2 + b], rows_stream));
 	}
+	// what follows on the main stream reads every row pass's output
+	if (nchunks > 1)
+		ARMA_HIP_CHECK(hipStreamWaitEvent(ctx->stream, ev[2 + ((nchunks - 1) & 1)], 0));
 }
 
 template <class T, int NX, int M = 2>

@@ -546,6 +546,57 @@ void dft_axis(std::vector<std::complex<double>>& a, const int s[3], int ax, int 
 
 }  // namespace
 
+// ACF_generator<T>::generate up to the auto-covariance
+// (src/generator/acf_generator.cc:130-215): the plain wave on
+// [-nwaves, nwaves] with 2*shape+1 points per dimension (or, without a shape,
+// doubled from 2 until its variance settles), times exp(-sum |alpha (t,x,y)|),
+// rescaled to the plain wave's variance.  s receives its extents.
+void arma_hip::acf_gen_wave(const arma_acf_gen_desc& d, std::vector<double>& we, int s[3]) {
+	int ws[3] = {2, 2, 2};
+	Wave w;
+	if (d.shape[0] > 0 && d.shape[1] > 0 && d.shape[2] > 0) {
+		for (int q = 0; q < 3; ++q) ws[q] = 2 * d.shape[q];
+		w = make_wave(d, ws);
+	} else {
+		// generate_optimal_wavy_surface (src/generator/acf_generator.cc:158-193)
+		double var0 = -1.0, var = -1.0;
+		while (ws[0] < 64 && ws[1] < 64 && ws[2] < 64) {
+			w = make_wave(d, ws);
+			var0 = var;
+			var = gsl_style_variance(w.v);
+			if (!(var0 < 0.0) && std::fabs(var - var0) < 1e-3) break;
+			for (int q = 0; q < 3; ++q) ws[q] *= 2;
+		}
+	}
+	// add_exponential_decay (src/generator/acf_generator.cc:196-215)
+	double step[3];
+	for (int q = 0; q < 3; ++q) step[q] = (d.nwaves[q] - (-d.nwaves[q])) / (w.s[q] - 1);
+	we.resize(w.v.size());
+	size_t q = 0;
+	for (int i = 0; i < w.s[0]; ++i)
+		for (int j = 0; j < w.s[1]; ++j)
+			for (int k = 0; k < w.s[2]; ++k, ++q) {
+				const double t = -d.nwaves[0] + step[0] * i, x = -d.nwaves[1] + step[1] * j,
+				             y = -d.nwaves[2] + step[2] * k;
+				we[q] = w.v[q] * std::exp(-(std::fabs(t * d.alpha[0]) + std::fabs(x * d.alpha[1]) +
+				                            std::fabs(y * d.alpha[2])));
+			}
+	const double f = std::sqrt(gsl_style_variance(w.v) / gsl_style_variance(we));
+	for (double& v : we) v *= f;
+	for (int r = 0; r < 3; ++r) s[r] = w.s[r];
+}
+
+// The first half of the auto-covariance in every dimension and its grid
+// (Grid{shape, nwaves}: spacing nwaves / (shape - 1)).
+void arma_hip::acf_gen_crop(const arma_acf_gen_desc& d, const int s[3], int h[3],
+                            int32_t shape_out[3], double delta_out[3]) {
+	for (int r = 0; r < 3; ++r) {
+		h[r] = s[r] / 2;
+		shape_out[r] = h[r];
+		delta_out[r] = h[r] > 1 ? d.nwaves[r] / double(h[r] - 1) : 0.0;
+	}
+}
+
 extern "C" {
 
 int arma_hip_acf(int32_t func, const double delta[3], const int32_t shape[3], double amplitude,
@@ -574,54 +625,20 @@ int arma_hip_acf_generate(const arma_acf_gen_desc* desc, double* out, size_t cap
 	if (!desc || !out || !shape_out || !delta_out) return ARMA_HIP_EINVAL;
 	if (desc->func < ARMA_WAVE_SIN || desc->func > ARMA_WAVE_STANDING) return ARMA_HIP_EINVAL;
 	try {
-		const arma_acf_gen_desc& d = *desc;
-		int ws[3] = {2, 2, 2};
-		Wave w;
-		if (d.shape[0] > 0 && d.shape[1] > 0 && d.shape[2] > 0) {
-			for (int q = 0; q < 3; ++q) ws[q] = 2 * d.shape[q];
-			w = make_wave(d, ws);
-		} else {
-			// generate_optimal_wavy_surface (src/generator/acf_generator.cc:158-193)
-			double var0 = -1.0, var = -1.0;
-			while (ws[0] < 64 && ws[1] < 64 && ws[2] < 64) {
-				w = make_wave(d, ws);
-				var0 = var;
-				var = gsl_style_variance(w.v);
-				if (!(var0 < 0.0) && std::fabs(var - var0) < 1e-3) break;
-				for (int q = 0; q < 3; ++q) ws[q] *= 2;
-			}
-		}
-		// add_exponential_decay (src/generator/acf_generator.cc:196-215)
-		double step[3];
-		for (int q = 0; q < 3; ++q) step[q] = (d.nwaves[q] - (-d.nwaves[q])) / (w.s[q] - 1);
-		std::vector<double> we(w.v.size());
-		size_t q = 0;
-		for (int i = 0; i < w.s[0]; ++i)
-			for (int j = 0; j < w.s[1]; ++j)
-				for (int k = 0; k < w.s[2]; ++k, ++q) {
-					const double t = -d.nwaves[0] + step[0] * i, x = -d.nwaves[1] + step[1] * j,
-					             y = -d.nwaves[2] + step[2] * k;
-					we[q] = w.v[q] * std::exp(-(std::fabs(t * d.alpha[0]) +
-					                            std::fabs(x * d.alpha[1]) +
-					                            std::fabs(y * d.alpha[2])));
-				}
-		const double f = std::sqrt(gsl_style_variance(w.v) / gsl_style_variance(we));
-		for (double& v : we) v *= f;
+		std::vector<double> we;
+		int s[3];
+		arma_hip::acf_gen_wave(*desc, we, s);
 		// auto_covariance (src/arma.cc:81-100)
 		std::vector<std::complex<double>> a(we.begin(), we.end());
-		for (int ax = 0; ax < 3; ++ax) dft_axis(a, w.s, ax, -1);
+		for (int ax = 0; ax < 3; ++ax) dft_axis(a, s, ax, -1);
 		for (auto& c : a) {
 			const double m = std::abs(c);
 			c = std::complex<double>(m * m, 0.0);
 		}
-		for (int ax = 0; ax < 3; ++ax) dft_axis(a, w.s, ax, +1);
+		for (int ax = 0; ax < 3; ++ax) dft_axis(a, s, ax, +1);
 		const double n = double(a.size());
 		int h[3];
-		for (int r = 0; r < 3; ++r) {
-			h[r] = w.s[r] / 2;
-			shape_out[r] = h[r];
-			delta_out[r] = h[r] > 1 ? d.nwaves[r] / double(h[r] - 1) : 0.0;
-		}
+		arma_hip::acf_gen_crop(*desc, s, h, shape_out, delta_out);
 		if (size_t(h[0]) * h[1] * h[2] > cap) {
 			arma_hip::set_last_error("ACF output buffer too small");
 			return ARMA_HIP_EINVAL;
@@ -630,7 +647,7 @@ int arma_hip_acf_generate(const arma_acf_gen_desc* desc, double* out, size_t cap
 			for (int j = 0; j < h[1]; ++j)
 				for (int k = 0; k < h[2]; ++k)
 					out[(size_t(i) * h[1] + j) * h[2] + k] =
-						a[(size_t(i) * w.s[1] + j) * w.s[2] + k].real() / n / n;
+						a[(size_t(i) * s[1] + j) * s[2] + k].real() / n / n;
 		return ARMA_HIP_OK;
 	} catch (const std::exception& e) {
 		arma_hip::set_last_error(e.what());

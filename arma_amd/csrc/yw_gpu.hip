@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 namespace arma_hip {
@@ -156,12 +157,48 @@ __device__ bool inverse(const YwArgs& y, DMat inv, DMat src) {
 	__syncthreads();
 	__shared__ int pk;
 	__shared__ int bad;
+	__shared__ double wbest[kYwThreads / 64];
+	__shared__ int wbidx[kYwThreads / 64];
 	if (threadIdx.x == 0) bad = 0;
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	for (int k = 0; k < n; ++k) {
+		// pivot: the first row i >= k of largest |lu(i, k)| (the host's
+		// sequential scan keeps a row only when a later one is strictly
+		// larger), as a lane scan, a wave reduction and a fold of the waves,
+		// each preferring the smaller row on ties
+		double bv = -1.0;
+		int bi = n;
+		for (int i = k + threadIdx.x; i < n; i += blockDim.x) {
+			const double v = fabs(lu[size_t(i) * n + k]);
+			if (v > bv) {
+				bv = v;
+				bi = i;
+			}
+		}
+		for (int off = 32; off > 0; off >>= 1) {
+			const double ov = __shfl_xor(bv, off);
+			const int oi = __shfl_xor(bi, off);
+			if (ov > bv || (ov == bv && oi < bi)) {
+				bv = ov;
+				bi = oi;
+			}
+		}
+		if (lane == 0) {
+			wbest[wave] = bv;
+			wbidx[wave] = bi;
+		}
+		__syncthreads();
 		if (threadIdx.x == 0) {
-			int p = k;
-			for (int i = k + 1; i < n; ++i)
-				if (fabs(lu[size_t(i) * n + k]) > fabs(lu[size_t(p) * n + k])) p = i;
+			int p = wbidx[0];
+			double pv = wbest[0];
+			for (int q = 1; q < int(blockDim.x) / 64; ++q)
+				if (wbest[q] > pv || (wbest[q] == pv && wbidx[q] < p)) {
+					pv = wbest[q];
+					p = wbidx[q];
+				}
+			// the scan never leaves a NaN diagonal (no value compares larger)
+			// and never picks a NaN row
+			if (p >= n || isnan(lu[size_t(k) * n + k])) p = k;
 			if (lu[size_t(p) * n + k] == 0.0) bad = 1;
 			pk = p;
 			y.piv[k] = p;
@@ -220,7 +257,7 @@ __device__ DMat pi_block(const YwArgs& y, double* base, int m) {
 	return DMat{base + y.pi_off[m], shell(m), 1};
 }
 
-__global__ __launch_bounds__(kYwThreads) void choi_kernel(YwArgs y) {
+__device__ void choi_body(const YwArgs& y) {
 	const int M = y.M;
 	double* Pl = y.pi[0];
 	double* Plp1 = y.pi[1];
@@ -299,6 +336,13 @@ __global__ __launch_bounds__(kYwThreads) void choi_kernel(YwArgs y) {
 	}
 }
 
+// One workgroup per model: the recursions of a batch's models run side by
+// side, each exactly as a single solve.
+__global__ __launch_bounds__(kYwThreads) void choi_kernel(const YwArgs* __restrict__ ys) {
+	const YwArgs y = ys[blockIdx.x];
+	choi_body(y);
+}
+
 // sigma^2 = gamma000 - sum phi * gamma over phi's shape, summed in the host
 // order (one lane).
 __global__ void varwn_kernel(const double* acf, int a1, int a2, const double* phi, int p0, int p1,
@@ -320,147 +364,209 @@ void push_points(std::vector<int>& v, int i, int j, int k) {
 
 }  // namespace
 
-void yule_walker_choi_device(arma_hip_ctx* ctx, const double* acf, int n, int max_order,
-                             double var_eps, bool determine, std::vector<double>& phi, int& l,
-                             double& varwn) {
-	const int M = std::min(max_order, n - 1);
-	ARMA_HIP_REQUIRE(M >= 2, ARMA_HIP_EINVAL, "AR order below 2 for Choi solver");
+namespace {
+
+// Host tables and device layout of one model's recursion.
+struct YwPlan {
+	int n = 0, M = 0, MP = 0;
+	double var_eps = 0.0, variance = 0.0;
+	bool determine = true;
+	std::vector<int> rows, cols, rows_off, cols_off;
+	std::vector<long long> phi_off, pi_off, theta_off;
+	std::vector<double> rho;
+	long long nphi = 0, npi = 0, nth = 0, S = 0;
+	size_t nd = 0, nll = 0, ni = 0;  // doubles, 8-byte offsets, ints
+	size_t bytes() const { return (nd * 8 + nll * 8 + ni * 4 + 255) / 256 * 256; }
+	size_t out_len() const { return size_t(M + 1) * (M + 1) * (M + 1) + 2; }
+};
+
+YwPlan plan_model(const double* acf, int n, int max_order, double var_eps, bool determine) {
+	YwPlan q;
+	q.n = n;
+	q.M = std::min(max_order, n - 1);
+	ARMA_HIP_REQUIRE(q.M >= 2, ARMA_HIP_EINVAL, "AR order below 2 for Choi solver");
 	ARMA_HIP_REQUIRE(acf[0] != 0.0, ARMA_HIP_EINVAL, "zero variance in the ACF");
-	const int MP = M + 2;
+	q.var_eps = var_eps;
+	q.determine = determine;
+	q.variance = acf[0];
+	const int MP = q.MP = q.M + 2;
 	// point tables up to M + 1
-	std::vector<int> rows, cols, rows_off(MP), cols_off(MP);
+	q.rows_off.resize(MP);
+	q.cols_off.resize(MP);
 	for (int a = 0; a < MP; ++a) {
-		rows_off[a] = int(rows.size() / 3);
-		push_points(rows, a, a, a);
+		q.rows_off[a] = int(q.rows.size() / 3);
+		push_points(q.rows, a, a, a);
 		for (int e = 1; e <= a; ++e) {
-			for (int q = 0; q < e; ++q) push_points(rows, a - e, a - q, a);
-			for (int q = 0; q < e; ++q) push_points(rows, a - e + q, a - e, a);
-			for (int q = 0; q < e; ++q) push_points(rows, a, a - e, a - q);
-			for (int q = 0; q < e; ++q) push_points(rows, a, a - e + q, a - e);
-			for (int q = 0; q < e; ++q) push_points(rows, a - q, a, a - e);
-			for (int q = 0; q < e; ++q) push_points(rows, a - e, a, a - e + q);
+			for (int r = 0; r < e; ++r) push_points(q.rows, a - e, a - r, a);
+			for (int r = 0; r < e; ++r) push_points(q.rows, a - e + r, a - e, a);
+			for (int r = 0; r < e; ++r) push_points(q.rows, a, a - e, a - r);
+			for (int r = 0; r < e; ++r) push_points(q.rows, a, a - e + r, a - e);
+			for (int r = 0; r < e; ++r) push_points(q.rows, a - r, a, a - e);
+			for (int r = 0; r < e; ++r) push_points(q.rows, a - e, a, a - e + r);
 		}
-		cols_off[a] = int(cols.size() / 3);
-		push_points(cols, 0, 0, 0);
+		q.cols_off[a] = int(q.cols.size() / 3);
+		push_points(q.cols, 0, 0, 0);
 		for (int e = 1; e <= a; ++e) {
-			for (int q = 0; q < e; ++q) push_points(cols, e, q, 0);
-			for (int q = 0; q < e; ++q) push_points(cols, e - q, e, 0);
-			for (int q = 0; q < e; ++q) push_points(cols, 0, e, q);
-			for (int q = 0; q < e; ++q) push_points(cols, 0, e - q, e);
-			for (int q = 0; q < e; ++q) push_points(cols, q, 0, e);
-			for (int q = 0; q < e; ++q) push_points(cols, e, 0, e - q);
+			for (int r = 0; r < e; ++r) push_points(q.cols, e, r, 0);
+			for (int r = 0; r < e; ++r) push_points(q.cols, e - r, e, 0);
+			for (int r = 0; r < e; ++r) push_points(q.cols, 0, e, r);
+			for (int r = 0; r < e; ++r) push_points(q.cols, 0, e - r, e);
+			for (int r = 0; r < e; ++r) push_points(q.cols, r, 0, e);
+			for (int r = 0; r < e; ++r) push_points(q.cols, e, 0, e - r);
 		}
 	}
 	// block offsets (in doubles)
-	std::vector<long long> phi_off(size_t(MP) * MP * MP, -1), pi_off(MP), theta_off(MP);
-	long long nphi = 0, npi = 0, nth = 0;
+	q.phi_off.assign(size_t(MP) * MP * MP, -1);
+	q.pi_off.resize(MP);
+	q.theta_off.resize(MP);
 	for (int p = 2; p < MP; ++p)
 		for (int a = 1; a < p; ++a)
 			for (int j = 0; p + j < MP; ++j) {
-				phi_off[(size_t(p) * MP + a) * MP + j] = nphi;
-				nphi += (long long)shell(a) * shell(p + j);
+				q.phi_off[(size_t(p) * MP + a) * MP + j] = q.nphi;
+				q.nphi += (long long)shell(a) * shell(p + j);
 			}
 	for (int m = 0; m < MP; ++m) {
-		pi_off[m] = npi;
-		npi += shell(m);
-		theta_off[m] = nth;
-		nth += (long long)shell(m) * shell(m);
+		q.pi_off[m] = q.npi;
+		q.npi += shell(m);
+		q.theta_off[m] = q.nth;
+		q.nth += (long long)shell(m) * shell(m);
 	}
-	const long long S = shell(M + 1);
+	q.S = shell(q.M + 1);
 	const size_t n3 = size_t(n) * n * n;
-	std::vector<double> rho(n3);
-	for (size_t q = 0; q < n3; ++q) rho[q] = acf[q] / acf[0];
-	// one device buffer: doubles first, then 8-byte index tables, then ints
-	const size_t nd = n3 + size_t(nphi) + 2 * size_t(npi) + size_t(nth) + size_t(S) * S * 5 +
-	                  size_t(M + 1) * (M + 1) * (M + 1) + 2;
-	const size_t nll = phi_off.size() + pi_off.size() + theta_off.size();
-	const size_t ni = rows.size() + cols.size() + 2 * MP + size_t(S) + 1;
-	unsigned char* buf = (unsigned char*)ctx->yw.get(nd * 8 + nll * 8 + ni * 4 + 64);
-	double* d = (double*)buf;
+	q.rho.resize(n3);
+	for (size_t e = 0; e < n3; ++e) q.rho[e] = acf[e] / acf[0];
+	q.nd = n3 + size_t(q.nphi) + 2 * size_t(q.npi) + size_t(q.nth) + size_t(q.S) * q.S * 5 +
+	       q.out_len();
+	q.nll = q.phi_off.size() + q.pi_off.size() + q.theta_off.size();
+	q.ni = q.rows.size() + q.cols.size() + 2 * MP + size_t(q.S) + 1;
+	return q;
+}
+
+// Lays the model out at device address dev (host image at img, same
+// offsets): doubles first, then the 8-byte index tables, then ints.  The
+// status word starts at 1 (not finished).
+YwArgs place_model(const YwPlan& q, unsigned char* dev, unsigned char* img) {
 	YwArgs y{};
-	y.n = n;
-	y.M = M;
-	y.var_eps = var_eps;
-	y.determine = determine ? 1 : 0;
-	y.variance = acf[0];
-	double* rho_d = d;
-	d += n3;
-	y.rho = rho_d;
-	y.phi = d;
-	d += nphi;
-	y.pi[0] = d;
-	d += npi;
-	y.pi[1] = d;
-	d += npi;
-	y.theta = d;
-	d += nth;
-	y.rsup11 = d;
-	d += S * S;
-	y.t0 = d;
-	d += S * S;
-	y.t1 = d;
-	d += S * S;
-	y.t2 = d;
-	d += S * S;
-	y.lu = d;
-	d += S * S;
-	y.out = d;
-	d += size_t(M + 1) * (M + 1) * (M + 1) + 2;
-	long long* ll = (long long*)d;
-	long long* phi_off_d = ll;
-	ll += phi_off.size();
-	long long* pi_off_d = ll;
-	ll += pi_off.size();
-	long long* theta_off_d = ll;
-	ll += theta_off.size();
-	int* ip = (int*)ll;
-	int* rows_d = ip;
-	ip += rows.size();
-	int* cols_d = ip;
-	ip += cols.size();
-	int* rows_off_d = ip;
-	ip += MP;
-	int* cols_off_d = ip;
-	ip += MP;
-	y.piv = ip;
-	ip += S;
-	y.status = ip;
-	y.phi_off = phi_off_d;
-	y.pi_off = pi_off_d;
-	y.theta_off = theta_off_d;
-	y.rows = rows_d;
-	y.cols = cols_d;
-	y.rows_off = rows_off_d;
-	y.cols_off = cols_off_d;
-	upload(ctx, rho_d, rho.data(), n3 * 8);
-	upload(ctx, phi_off_d, phi_off.data(), phi_off.size() * 8);
-	upload(ctx, pi_off_d, pi_off.data(), pi_off.size() * 8);
-	upload(ctx, theta_off_d, theta_off.data(), theta_off.size() * 8);
-	upload(ctx, rows_d, rows.data(), rows.size() * 4);
-	upload(ctx, cols_d, cols.data(), cols.size() * 4);
-	upload(ctx, rows_off_d, rows_off.data(), MP * 4);
-	upload(ctx, cols_off_d, cols_off.data(), MP * 4);
+	y.n = q.n;
+	y.M = q.M;
+	y.var_eps = q.var_eps;
+	y.determine = q.determine ? 1 : 0;
+	y.variance = q.variance;
+	size_t at = 0;  // bytes from dev
+	auto take = [&](size_t bytes) {
+		const size_t r = at;
+		at += bytes;
+		return r;
+	};
+	const size_t n3 = q.rho.size();
+	const size_t o_rho = take(n3 * 8);
+	std::memcpy(img + o_rho, q.rho.data(), n3 * 8);
+	y.rho = (const double*)(dev + o_rho);
+	y.phi = (double*)(dev + take(size_t(q.nphi) * 8));
+	y.pi[0] = (double*)(dev + take(size_t(q.npi) * 8));
+	y.pi[1] = (double*)(dev + take(size_t(q.npi) * 8));
+	y.theta = (double*)(dev + take(size_t(q.nth) * 8));
+	const size_t SS = size_t(q.S) * q.S * 8;
+	y.rsup11 = (double*)(dev + take(SS));
+	y.t0 = (double*)(dev + take(SS));
+	y.t1 = (double*)(dev + take(SS));
+	y.t2 = (double*)(dev + take(SS));
+	y.lu = (double*)(dev + take(SS));
+	y.out = (double*)(dev + take(q.out_len() * 8));
+	auto put_ll = [&](const std::vector<long long>& v) {
+		const size_t o = take(v.size() * 8);
+		std::memcpy(img + o, v.data(), v.size() * 8);
+		return (const long long*)(dev + o);
+	};
+	y.phi_off = put_ll(q.phi_off);
+	y.pi_off = put_ll(q.pi_off);
+	y.theta_off = put_ll(q.theta_off);
+	auto put_i = [&](const std::vector<int>& v) {
+		const size_t o = take(v.size() * 4);
+		std::memcpy(img + o, v.data(), v.size() * 4);
+		return (const int*)(dev + o);
+	};
+	y.rows = put_i(q.rows);
+	y.cols = put_i(q.cols);
+	y.rows_off = put_i(q.rows_off);
+	y.cols_off = put_i(q.cols_off);
+	y.piv = (int*)(dev + take(size_t(q.S) * 4));
+	const size_t o_st = take(4);
 	const int unset = 1;
-	upload(ctx, y.status, &unset, 4);
+	std::memcpy(img + o_st, &unset, 4);
+	y.status = (int*)(dev + o_st);
+	return y;
+}
+
+}  // namespace
+
+void yule_walker_choi_device_many(arma_hip_ctx* ctx, const std::vector<YwProblem>& probs,
+                                  std::vector<YwSolution>& sols) {
+	const size_t nm = probs.size();
+	sols.assign(nm, YwSolution{});
+	if (nm == 0) return;
+	std::vector<YwPlan> plans;
+	plans.reserve(nm);
+	std::vector<size_t> base(nm);
+	size_t total = 0;
+	for (size_t i = 0; i < nm; ++i) {
+		const YwProblem& pr = probs[i];
+		plans.push_back(plan_model(pr.acf, pr.n, pr.max_order, pr.var_eps, pr.determine));
+		base[i] = total;
+		total += plans.back().bytes();
+	}
+	const size_t args_at = total;
+	total += sizeof(YwArgs) * nm;
+	unsigned char* dev = (unsigned char*)ctx->yw.get(total);
+	std::vector<unsigned char> img(total, 0);
+	std::vector<YwArgs> args(nm);
+	for (size_t i = 0; i < nm; ++i) args[i] = place_model(plans[i], dev + base[i], img.data() + base[i]);
+	std::memcpy(img.data() + args_at, args.data(), sizeof(YwArgs) * nm);
+	upload(ctx, dev, img.data(), total);
 	{
 		KernelTimer timer(ctx, "yule_walker_choi");
-		choi_kernel<<<1, kYwThreads, 0, ctx->stream>>>(y);
+		choi_kernel<<<int(nm), kYwThreads, 0, ctx->stream>>>((const YwArgs*)(dev + args_at));
 		ARMA_HIP_CHECK(hipGetLastError());
 	}
-	int status = 0;
-	ARMA_HIP_CHECK(hipMemcpyAsync(&status, y.status, 4, hipMemcpyDeviceToHost, ctx->stream));
+	// every model's status and results come back with one copy of the
+	// whole buffer's result words
+	std::vector<int> status(nm);
+	std::vector<std::vector<double>> tails(nm);
+	for (size_t i = 0; i < nm; ++i) {
+		tails[i].resize(plans[i].out_len());
+		ARMA_HIP_CHECK(hipMemcpyAsync(&status[i], args[i].status, 4, hipMemcpyDeviceToHost,
+		                              ctx->stream));
+		ARMA_HIP_CHECK(hipMemcpyAsync(tails[i].data(), args[i].out, tails[i].size() * 8,
+		                              hipMemcpyDeviceToHost, ctx->stream));
+	}
 	ARMA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-	ARMA_HIP_REQUIRE(status == ARMA_HIP_OK, ARMA_HIP_ERUNTIME, "getrf error");
-	// l, varwn, then the coefficients
-	std::vector<double> tail(size_t(M + 1) * (M + 1) * (M + 1) + 2);
-	ARMA_HIP_CHECK(hipMemcpyAsync(tail.data(), y.out, tail.size() * 8, hipMemcpyDeviceToHost,
-	                              ctx->stream));
-	ARMA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-	l = int(tail[0]);
-	ARMA_HIP_REQUIRE(l >= 2 && l <= M, ARMA_HIP_ERUNTIME, "Choi solver returned no order");
-	const size_t L3 = size_t(l) * l * l;
-	varwn = tail[1];
-	phi.assign(tail.begin() + 2, tail.begin() + 2 + L3);
+	for (size_t i = 0; i < nm; ++i) {
+		YwSolution& r = sols[i];
+		if (status[i] != ARMA_HIP_OK) {
+			r.status = ARMA_HIP_ERUNTIME;  // getrf: a zero pivot
+			continue;
+		}
+		r.l = int(tails[i][0]);
+		if (!(r.l >= 2 && r.l <= plans[i].M)) {
+			r.status = ARMA_HIP_ERUNTIME;
+			continue;
+		}
+		r.status = ARMA_HIP_OK;
+		r.varwn = tails[i][1];
+		const size_t L3 = size_t(r.l) * r.l * r.l;
+		r.phi.assign(tails[i].begin() + 2, tails[i].begin() + 2 + L3);
+	}
+}
+
+void yule_walker_choi_device(arma_hip_ctx* ctx, const double* acf, int n, int max_order,
+                             double var_eps, bool determine, std::vector<double>& phi, int& l,
+                             double& varwn) {
+	std::vector<YwSolution> sol;
+	yule_walker_choi_device_many(ctx, {YwProblem{acf, n, max_order, var_eps, determine}}, sol);
+	ARMA_HIP_REQUIRE(sol[0].status == ARMA_HIP_OK, ARMA_HIP_ERUNTIME, "getrf error");
+	l = sol[0].l;
+	varwn = sol[0].varwn;
+	phi = std::move(sol[0].phi);
 }
 
 double white_noise_variance_device(arma_hip_ctx* ctx, const double* acf, const int32_t acf_shape[3],

@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <iomanip>
 #include <iostream>
 #include <limits>
 #include <random>
@@ -314,16 +315,11 @@ void AR_model<T>::read(std::istream& in) {
 					 }
 					 g.beta[0] = beta[0];
 					 g.beta[1] = beta[1];
-					 std::vector<double> buf(size_t(1) << 21);
-					 int32_t so[3];
-					 double de[3];
-					 check(arma_hip_acf_generate(&g, buf.data(), buf.size(), so, de));
-					 for (int k = 0; k < 3; ++k) {
-						 acf_shape_[k] = so[k];
-						 acf_grid_.npoints[k] = so[k];
-						 acf_grid_.length[k] = nwaves[k];
-					 }
-					 acf_.assign(buf.begin(), buf.begin() + size_t(so[0]) * so[1] * so[2]);
+					 // generated in prepare(): its auto-covariance runs on the GPU
+					 // when a generation follows, on the host for a dry run
+					 acf_gen_ = g;
+					 have_acf_gen_ = true;
+					 acf_.clear();
 					 have_acf = true;
 					 return;
 				 }
@@ -509,7 +505,24 @@ void AR_model<T>::prepare() {
 
 template <class T>
 void AR_model<T>::prepare(bool on_device) {
-	// src/generator/basic_arma_model.cc:51-97 with the closed-form ACF
+	// src/generator/basic_arma_model.cc:51-97 with the closed-form ACF, or
+	// ACF_generator's (src/generator/basic_arma_model.cc:57)
+	if (have_acf_gen_ && acf_.empty()) {
+		std::vector<double> buf(size_t(1) << 21);
+		int32_t so[3];
+		double de[3];
+		if (on_device)
+			check(arma_hip_acf_generate_device(ctx_->get(), &acf_gen_, buf.data(), buf.size(), so,
+			                                   de));
+		else
+			check(arma_hip_acf_generate(&acf_gen_, buf.data(), buf.size(), so, de));
+		for (int k = 0; k < 3; ++k) {
+			acf_shape_[k] = so[k];
+			acf_grid_.npoints[k] = so[k];
+			acf_grid_.length[k] = acf_gen_.nwaves[k];
+		}
+		acf_.assign(buf.begin(), buf.begin() + size_t(so[0]) * so[1] * so[2]);
+	}
 	if (!(order_[0] > 0 && order_[1] > 0 && order_[2] > 0)) order_ = acf_shape_;
 	for (int d = 0; d < 3; ++d)
 		this->outgrid_.length[d] = acf_grid_.delta(d) * this->outgrid_.num_patches(d) * 2.0 * M_PI;
@@ -656,6 +669,27 @@ void write_big_endian(std::ostream& out, const std::vector<T>& v) {
 	out.write((const char*)buf.data(), std::streamsize(buf.size()));
 }
 
+// Blitz++ text form of an array of rank >= 2, as `std::ofstream(name) << a`
+// writes it (src/arma_driver.cc:33,54; Blitz++ operator<< for Array<T,N>):
+// the index ranges "(0,n0-1) x (0,n1-1) x ...", a newline, "[ ", every
+// element right-aligned in 9 columns followed by a space, a newline and two
+// spaces after every 7 elements, then "]" and a newline.  Stream defaults
+// otherwise (6 significant digits).
+template <class T, int N>
+void write_blitz(std::ostream& out, const Array<T, N>& a) {
+	for (int d = 0; d < N; ++d) {
+		out << '(' << 0 << ',' << a.shape[d] - 1 << ')';
+		if (d != N - 1) out << " x ";
+	}
+	out << '\n' << "[ ";
+	size_t p = 0;
+	for (const T& v : a.data) {
+		out << std::setw(9) << v << ' ';
+		if (++p % 7 == 0) out << '\n' << "  ";
+	}
+	out << ']' << '\n';
+}
+
 }  // namespace
 
 template <class T>
@@ -670,19 +704,30 @@ void ARMA_driver<T>::write_all() {
 		std::ofstream out("zeta.bin", std::ios::binary);
 		write_big_endian(out, zeta_.data);
 	}
+	// text outputs keep the stream's default format (6 significant digits),
+	// as the reference's writers do (src/bits/write_csv.hh:13-80)
+	if (f.isset(Output_flags::Blitz)) {
+		std::ofstream out("zeta");
+		write_blitz(out, zeta_);
+	}
 	if (f.isset(Output_flags::CSV)) {
 		std::ofstream out("zeta.csv");
-		out.precision(std::numeric_limits<T>::max_digits10);
 		out << "t,x,y,z\n";
 		for (int i = 0; i < nt; ++i)
 			for (int j = 0; j < nx; ++j)
 				for (int k = 0; k < ny; ++k)
 					out << i << ',' << g.delta(1) * j << ',' << g.delta(2) * k << ','
 					    << zeta_.data[(size_t(i) * nx + j) * ny + k] << '\n';
+	}
+	// velocity potentials (src/arma_driver.cc:50-69; no binary form)
+	if (f.isset(Output_flags::Blitz)) {
+		std::ofstream out("phi");
+		write_blitz(out, vpotentials_);
+	}
+	if (f.isset(Output_flags::CSV)) {
 		const auto& dom = solver_->domain();
 		const int ntp = vpotentials_.shape[0], nz = vpotentials_.shape[1];
 		std::ofstream po("phi.csv");
-		po.precision(std::numeric_limits<T>::max_digits10);
 		po << "t,z,x,y,phi\n";
 		for (int i = 0; i < ntp; ++i)
 			for (int j = 0; j < nz; ++j)
@@ -691,14 +736,6 @@ void ARMA_driver<T>::write_all() {
 						po << dom(i, 0) << ',' << dom(j, 1) << ',' << g.delta(1) * k << ','
 						   << g.delta(2) * l << ','
 						   << vpotentials_.data[((size_t(i) * nz + j) * nx + k) * ny + l] << '\n';
-	}
-	if (f.isset(Output_flags::Blitz)) {
-		// Blitz++ text layout: the extents, then the values in brackets
-		std::ofstream out("zeta");
-		out.precision(std::numeric_limits<T>::max_digits10);
-		out << "(0," << nt - 1 << ") x (0," << nx - 1 << ") x (0," << ny - 1 << ")\n[ ";
-		for (size_t q = 0; q < zeta_.data.size(); ++q) out << zeta_.data[q] << ' ';
-		out << "]\n";
 	}
 }
 

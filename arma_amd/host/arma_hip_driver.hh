@@ -174,6 +174,9 @@ private:
 	std::vector<double> acf_;
 	Shape3D acf_shape_{0, 0, 0};
 	Grid<3> acf_grid_;
+	// an acf block without grid = : ACF_generator's parameters, the ACF made in prepare()
+	arma_acf_gen_desc acf_gen_{};
+	bool have_acf_gen_ = false;
 	Shape3D order_{0, 0, 0};
 	Shape3D partition_{0, 0, 0};
 	AR_algorithm algorithm_ = AR_algorithm::Choi;

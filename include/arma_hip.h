@@ -270,6 +270,14 @@ typedef struct arma_acf_gen_desc {
 int arma_hip_acf_generate(const arma_acf_gen_desc* desc, double* out, size_t cap,
                           int32_t shape_out[3], double delta_out[3]);
 
+/* arma_hip_acf_generate with the auto-covariance on the GPU of ctx: the
+   circular auto-covariance (1/n) sum_x w(x) w((x - d) mod s) of the decayed
+   wave, which real(IFFT(|FFT(w)|^2))/n^2 equals, summed directly in FP64 for
+   every lag d of the first half (src/arma.cc:81-100).  Host inputs and
+   outputs; waits for the context stream. */
+int arma_hip_acf_generate_device(arma_hip_ctx* ctx, const arma_acf_gen_desc* desc, double* out,
+                                 size_t cap, int32_t shape_out[3], double delta_out[3]);
+
 /* Closed-form ACF on a grid of shape points with spacing delta
    (src/acf.hh:18-62 via src/bits/acf_wrapper.cc:41-70). */
 int arma_hip_acf(int32_t func, const double delta[3], const int32_t shape[3], double amplitude,
@@ -302,6 +310,19 @@ int arma_hip_yule_walker_choi_device(arma_hip_ctx* ctx, const double* acf, int32
 int arma_hip_white_noise_variance_device(arma_hip_ctx* ctx, const double* acf,
                                          const int32_t acf_shape[3], const double* phi,
                                          const int32_t phi_shape[3], double* out);
+/* Many Choi solves in one launch, one workgroup per model (the models of a
+   batch of surfaces; each result equals the single solve's to the last
+   bit).  Model i reads the n[i]^3 cube at acf + acf_off[i] and writes at
+   most max_order[i]^3 doubles at phi_out + phi_off[i]; order_out[3i..3i+2]
+   receives its shape, varwn_solver[i] (optional) the solver's variance and
+   status[i] ARMA_HIP_OK or the model's own error. */
+int arma_hip_yule_walker_choi_device_many(arma_hip_ctx* ctx, int32_t count, const double* acf,
+                                          const int64_t* acf_off, const int32_t* n,
+                                          const int32_t* max_order, double var_eps,
+                                          int32_t determine_order, int32_t chop,
+                                          double chop_eps, double* phi_out,
+                                          const int64_t* phi_off, int32_t* order_out,
+                                          double* varwn_solver, int32_t* status);
 
 /* validate_process (src/arma.cc:15-70): the roots of 1 - phi_1 x - phi_2 x^2
    - ... over the row-major coefficients (phi(0,0,0) replaced by the 1) must

@@ -239,37 +239,56 @@ int orc_noise_sequential_f(const orc_mt_params* p, uint32_t seed, float stddev,
    partition order, bounds clipped to the grid); per-partition noise with a
    fresh normal_distribution: src/generator/ar_model_wn.cc:3-18.  Stream p is
    assigned to partition p (deterministic variant of :153-186). */
-int orc_noise_partitioned_d(const orc_mt_params* p, const uint32_t* seeds, int n_mts,
-                            const int shape[3], const int part[3], double stddev,
-                            double* out) {
+static int noise_partitioned(const orc_mt_params* p, const uint32_t* seeds, int n_mts,
+                             const int shape[3], const int part[3], double stddev,
+                             double* out, int nthreads) {
 	int np[3];
 	for (int d = 0; d < 3; ++d) np[d] = shape[d] / part[d] + (shape[d] % part[d] ? 1 : 0);
 	const int ntotal = np[0] * np[1] * np[2];
 	if (n_mts < ntotal) return -1;
-	orc_mt* g = (orc_mt*)malloc(sizeof(orc_mt));
-	if (!g) return -2;
-	int pid = 0;
-	for (int a = 0; a < np[0]; ++a)
-	for (int b = 0; b < np[1]; ++b)
-	for (int c = 0; c < np[2]; ++c, ++pid) {
-		const int lo[3] = {a * part[0], b * part[1], c * part[2]};
-		int hi[3];
-		const int ijk[3] = {a, b, c};
-		for (int d = 0; d < 3; ++d) {
-			int u = (ijk[d] + 1) * part[d];
-			hi[d] = (u < shape[d] ? u : shape[d]);
+	int rc = 0;
+	/* partitions are independent (one stream each): the OpenMP variant of
+	   src/generator/ar_model_parallel.cc:181-186 with a fixed assignment */
+#pragma omp parallel num_threads(nthreads > 0 ? nthreads : 1) if (nthreads > 1)
+	{
+		orc_mt* g = (orc_mt*)malloc(sizeof(orc_mt));
+#pragma omp for schedule(dynamic)
+		for (int pid = 0; pid < ntotal; ++pid) {
+			const int ijk[3] = {pid / (np[1] * np[2]), (pid / np[2]) % np[1], pid % np[2]};
+			int lo[3], hi[3];
+			for (int d = 0; d < 3; ++d) {
+				lo[d] = ijk[d] * part[d];
+				int u = (ijk[d] + 1) * part[d];
+				hi[d] = (u < shape[d] ? u : shape[d]);
+			}
+			if (!g || orc_mt_seed(g, &p[pid], seeds ? seeds[pid] : 0u)) {
+#pragma omp atomic write
+				rc = -3;
+				continue;
+			}
+			orc_normal_d st = {0, 0.0};
+			for (int t = lo[0]; t < hi[0]; ++t)
+			for (int x = lo[1]; x < hi[1]; ++x)
+			for (int y = lo[2]; y < hi[2]; ++y) {
+				out[((size_t)t * shape[1] + x) * shape[2] + y] =
+					orc_normal_next_d(g, &st, 0.0, stddev);
+			}
 		}
-		if (orc_mt_seed(g, &p[pid], seeds ? seeds[pid] : 0u)) { free(g); return -3; }
-		orc_normal_d st = {0, 0.0};
-		for (int t = lo[0]; t < hi[0]; ++t)
-		for (int x = lo[1]; x < hi[1]; ++x)
-		for (int y = lo[2]; y < hi[2]; ++y) {
-			out[((size_t)t * shape[1] + x) * shape[2] + y] =
-				orc_normal_next_d(g, &st, 0.0, stddev);
-		}
+		free(g);
 	}
-	free(g);
-	return 0;
+	return rc;
+}
+
+int orc_noise_partitioned_d(const orc_mt_params* p, const uint32_t* seeds, int n_mts,
+                            const int shape[3], const int part[3], double stddev,
+                            double* out) {
+	return noise_partitioned(p, seeds, n_mts, shape, part, stddev, out, 1);
+}
+
+int orc_noise_partitioned_d_omp(const orc_mt_params* p, const uint32_t* seeds, int n_mts,
+                                const int shape[3], const int part[3], double stddev,
+                                double* out, int nthreads) {
+	return noise_partitioned(p, seeds, n_mts, shape, part, stddev, out, nthreads);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -788,6 +807,32 @@ void orc_ar_generate_dag(double* zeta, const int shape[3], const double* phi,
 	}
 }
 
+/* The partition DAG with OpenMP (src/generator/ar_model_parallel.cc:151-196
+   runs every partition whose 7 predecessors are done): partitions of one
+   anti-diagonal level a + b + c = L depend only on lower levels, so each
+   level runs as one parallel loop.  Every point is computed by the same
+   loop body, so the result equals orc_ar_generate_dag bit for bit. */
+void orc_ar_generate_dag_omp(double* zeta, const int shape[3], const double* phi,
+                             const int pshape[3], const int part[3], int nthreads) {
+	int np[3];
+	for (int d = 0; d < 3; ++d) np[d] = shape[d] / part[d] + (shape[d] % part[d] ? 1 : 0);
+	const int ntotal = np[0] * np[1] * np[2];
+	for (int L = 0; L <= np[0] + np[1] + np[2] - 3; ++L) {
+#pragma omp parallel for schedule(dynamic) num_threads(nthreads > 0 ? nthreads : 1)
+		for (int pid = 0; pid < ntotal; ++pid) {
+			const int ijk[3] = {pid / (np[1] * np[2]), (pid / np[2]) % np[1], pid % np[2]};
+			if (ijk[0] + ijk[1] + ijk[2] != L) continue;
+			int lo[3], hi[3];
+			for (int d = 0; d < 3; ++d) {
+				lo[d] = ijk[d] * part[d];
+				int u = (ijk[d] + 1) * part[d];
+				hi[d] = (u < shape[d] ? u : shape[d]) - 1;
+			}
+			orc_ar_generate(zeta, shape, phi, pshape, lo, hi);
+		}
+	}
+}
+
 /* ------------------------------------------------------------------------ */
 /* GSL gsl_stats_variance (statistics/mean_source.c, variance_source.c):    */
 /* running long-double mean, then running long-double mean of squared        */
@@ -1112,7 +1157,8 @@ static void ha_multiplier(double zt, double zx, double zy, double* re, double* i
    (1/kmin_x, 1/kmin_y). */
 static int potential(const double* zeta, const int shape[3], const double delta[3],
                      double depth, const double lbound[2], const double ubound[2],
-                     const int npoints[2], double* out, double* wnmax_out, int solver) {
+                     const int npoints[2], double* out, double* wnmax_out, int solver,
+                     int nthreads) {
 	const int nx = shape[1], ny = shape[2];
 	const size_t s = (size_t)nx * ny;
 	if (lbound[1] < -depth) return -1;
@@ -1123,9 +1169,6 @@ static int potential(const double* zeta, const int shape[3], const double delta[
 	double* zy = (double*)malloc(sizeof(double) * s);
 	double* fr = (double*)malloc(sizeof(double) * s);
 	double* fi = (double*)malloc(sizeof(double) * s);
-	double* wr = (double*)malloc(sizeof(double) * s);
-	double* wi = (double*)malloc(sizeof(double) * s);
-	double* win = (double*)malloc(sizeof(double) * s);
 	for (int i = 0; i < npoints[0]; ++i) {
 		const double tval = lbound[0] + dtd * i;
 		const int idx_t = (int)tval;
@@ -1143,30 +1186,46 @@ static int potential(const double* zeta, const int shape[3], const double delta[
 		const double wx = 1.0 / kr[0], wy = 1.0 / kr[1];
 		if (wnmax_out) { wnmax_out[2 * i] = wx; wnmax_out[2 * i + 1] = wy; }
 		orc_fft2(fr, fi, nx, ny, -1);
-		for (int j = 0; j < npoints[1]; ++j) {
-			const double z = lbound[1] + dzd * j;
-			orc_window(nx, ny, wx, wy, z, depth, win);
-			for (size_t q = 0; q < s; ++q) { wr[q] = fr[q] * win[q]; wi[q] = fi[q] * win[q]; }
-			orc_fft2(wr, wi, nx, ny, +1);
-			double* res = out + ((size_t)i * npoints[1] + j) * s;
-			for (size_t q = 0; q < s; ++q) res[q] = wr[q] / (double)s;
-			orc_interp_edges(res, nx, ny);
+		/* the levels of one slice are independent: OpenMP over z as in
+		   src/velocity/basic_solver.cc:55-57 */
+#pragma omp parallel num_threads(nthreads > 0 ? nthreads : 1) if (nthreads > 1)
+		{
+			double* wr = (double*)malloc(sizeof(double) * s);
+			double* wi = (double*)malloc(sizeof(double) * s);
+			double* win = (double*)malloc(sizeof(double) * s);
+#pragma omp for schedule(static)
+			for (int j = 0; j < npoints[1]; ++j) {
+				const double z = lbound[1] + dzd * j;
+				orc_window(nx, ny, wx, wy, z, depth, win);
+				for (size_t q = 0; q < s; ++q) { wr[q] = fr[q] * win[q]; wi[q] = fi[q] * win[q]; }
+				orc_fft2(wr, wi, nx, ny, +1);
+				double* res = out + ((size_t)i * npoints[1] + j) * s;
+				for (size_t q = 0; q < s; ++q) res[q] = wr[q] / (double)s;
+				orc_interp_edges(res, nx, ny);
+			}
+			free(wr); free(wi); free(win);
 		}
 	}
-	free(zt); free(zx); free(zy); free(fr); free(fi); free(wr); free(wi); free(win);
+	free(zt); free(zx); free(zy); free(fr); free(fi);
 	return 0;
 }
 
 int orc_linear_potential(const double* zeta, const int shape[3], const double delta[3],
                          double depth, const double lbound[2], const double ubound[2],
                          const int npoints[2], double* out, double* wnmax_out) {
-	return potential(zeta, shape, delta, depth, lbound, ubound, npoints, out, wnmax_out, 0);
+	return potential(zeta, shape, delta, depth, lbound, ubound, npoints, out, wnmax_out, 0, 1);
+}
+
+int orc_linear_potential_omp(const double* zeta, const int shape[3], const double delta[3],
+                             double depth, const double lbound[2], const double ubound[2],
+                             const int npoints[2], double* out, int nthreads) {
+	return potential(zeta, shape, delta, depth, lbound, ubound, npoints, out, NULL, 0, nthreads);
 }
 
 int orc_high_amplitude_potential(const double* zeta, const int shape[3], const double delta[3],
                                  double depth, const double lbound[2], const double ubound[2],
                                  const int npoints[2], double* out, double* wnmax_out) {
-	return potential(zeta, shape, delta, depth, lbound, ubound, npoints, out, wnmax_out, 1);
+	return potential(zeta, shape, delta, depth, lbound, ubound, npoints, out, wnmax_out, 1, 1);
 }
 
 /* ------------------------------------------------------------------------ */

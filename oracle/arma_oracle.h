@@ -62,6 +62,11 @@ int orc_noise_sequential_f(const orc_mt_params* p, uint32_t seed, float stddev,
 int orc_noise_partitioned_d(const orc_mt_params* p, const uint32_t* seeds, int n_mts,
                             const int shape[3], const int part[3], double stddev,
                             double* out);
+/* The same with the partitions spread over nthreads OpenMP threads (the
+   all-core CPU baseline; the result does not depend on nthreads). */
+int orc_noise_partitioned_d_omp(const orc_mt_params* p, const uint32_t* seeds, int n_mts,
+                                const int shape[3], const int part[3], double stddev,
+                                double* out, int nthreads);
 
 /* ---- ACF, Yule-Walker, AR ---- */
 
@@ -88,6 +93,8 @@ void orc_ar_generate_f(float* zeta, const int shape[3], const float* phi,
                        const int pshape[3], const int lo[3], const int hi[3]);
 void orc_ar_generate_dag(double* zeta, const int shape[3], const double* phi,
                          const int pshape[3], const int part[3]);
+void orc_ar_generate_dag_omp(double* zeta, const int shape[3], const double* phi,
+                             const int pshape[3], const int part[3], int nthreads);
 
 double orc_gsl_variance_d(const double* data, size_t n);
 float orc_gsl_variance_f(const float* data, size_t n);
@@ -108,6 +115,9 @@ void orc_interp_edges(double* res, int nx, int ny);
 int orc_linear_potential(const double* zeta, const int shape[3], const double delta[3],
                          double depth, const double lbound[2], const double ubound[2],
                          const int npoints[2], double* out, double* wnmax_out);
+int orc_linear_potential_omp(const double* zeta, const int shape[3], const double delta[3],
+                             double depth, const double lbound[2], const double ubound[2],
+                             const int npoints[2], double* out, int nthreads);
 
 /* ---- velocity potential (high-amplitude solver) ---- */
 

@@ -53,6 +53,7 @@ def lib():
         L.orc_noise_sequential_f.argtypes = [C.POINTER(MtParams), C.c_uint32, C.c_float, fp, C.c_size_t]
         L.orc_noise_partitioned_d.argtypes = [C.POINTER(MtParams), C.c_void_p, C.c_int, ip3, ip3,
                                               C.c_double, dp]
+        L.orc_noise_partitioned_d_omp.argtypes = L.orc_noise_partitioned_d.argtypes + [C.c_int]
         L.orc_acf.argtypes = [C.c_int, C.c_double * 3, ip3, C.c_double, C.c_double,
                               C.c_double * 3, C.c_double * 2, dp]
         L.orc_yw_choi.argtypes = [dp, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double,
@@ -65,6 +66,7 @@ def lib():
         L.orc_ar_generate.argtypes = [dp, ip3, dp, ip3, ip3, ip3]
         L.orc_ar_generate_f.argtypes = [fp, ip3, fp, ip3, ip3, ip3]
         L.orc_ar_generate_dag.argtypes = [dp, ip3, dp, ip3, ip3]
+        L.orc_ar_generate_dag_omp.argtypes = [dp, ip3, dp, ip3, ip3, C.c_int]
         L.orc_gsl_variance_d.argtypes = [dp, C.c_size_t]
         L.orc_gsl_variance_d.restype = C.c_double
         L.orc_gsl_variance_f.argtypes = [fp, C.c_size_t]
@@ -82,6 +84,9 @@ def lib():
                                            C.c_double * 2, C.c_int * 2, dp, C.c_void_p]
         L.orc_derivative_xy.argtypes = [dp, ip3, C.c_double, C.c_int, C.c_int, dp]
         L.orc_high_amplitude_potential.argtypes = L.orc_linear_potential.argtypes
+        L.orc_linear_potential_omp.argtypes = [dp, ip3, C.c_double * 3, C.c_double,
+                                               C.c_double * 2, C.c_double * 2, C.c_int * 2, dp,
+                                               C.c_int]
         _lib = L
     return _lib
 
@@ -114,14 +119,20 @@ def noise_sequential(params, seed, stddev, n, dtype=np.float64):
     return out
 
 
-def noise_partitioned(plist, seeds, shape, part, stddev):
+def noise_partitioned(plist, seeds, shape, part, stddev, threads=1):
+    """Partitioned noise; threads > 1 spreads the partitions over OpenMP
+    threads (same result)."""
     out = np.empty(tuple(shape), np.float64)
     arr = params_array(plist)
     sarr = None
     if seeds is not None:
         sarr = np.ascontiguousarray(seeds, dtype=np.uint32)
-    rc = lib().orc_noise_partitioned_d(arr, None if sarr is None else sarr.ctypes.data,
-                                       len(plist), _i3(shape), _i3(part), stddev, out.reshape(-1))
+    args = (arr, None if sarr is None else sarr.ctypes.data, len(plist), _i3(shape), _i3(part),
+            stddev, out.reshape(-1))
+    if threads > 1:
+        rc = lib().orc_noise_partitioned_d_omp(*args, threads)
+    else:
+        rc = lib().orc_noise_partitioned_d(*args)
     if rc != 0:
         raise RuntimeError(f"orc_noise_partitioned_d failed: {rc}")
     return out
@@ -189,10 +200,17 @@ def ar_generate(eps, phi, lo=None, hi=None):
     return z
 
 
-def ar_generate_dag(eps, phi, part):
+def ar_generate_dag(eps, phi, part, threads=1):
+    """The recurrence over the partition DAG; threads > 1 runs each
+    anti-diagonal level of partitions in parallel (same result)."""
     z = np.ascontiguousarray(eps, np.float64).copy()
     ph = np.ascontiguousarray(phi, np.float64)
-    lib().orc_ar_generate_dag(z.reshape(-1), _i3(z.shape), ph.reshape(-1), _i3(ph.shape), _i3(part))
+    if threads > 1:
+        lib().orc_ar_generate_dag_omp(z.reshape(-1), _i3(z.shape), ph.reshape(-1), _i3(ph.shape),
+                                      _i3(part), threads)
+    else:
+        lib().orc_ar_generate_dag(z.reshape(-1), _i3(z.shape), ph.reshape(-1), _i3(ph.shape),
+                                  _i3(part))
     return z
 
 
@@ -266,6 +284,20 @@ def linear_potential(zeta, delta, depth, lbound, ubound, npoints, return_wnmax=F
         raise ValueError("min(z) is less than -h")
     if return_wnmax:
         return out, wn.reshape(-1, 2)
+    return out
+
+
+def linear_potential_omp(zeta, delta, depth, lbound, ubound, npoints, threads):
+    """The linear potential with the levels of each slice spread over OpenMP
+    threads (same result as linear_potential)."""
+    z = np.ascontiguousarray(zeta, np.float64)
+    out = np.empty((npoints[0], npoints[1], z.shape[1], z.shape[2]), np.float64)
+    rc = lib().orc_linear_potential_omp(z.reshape(-1), _i3(z.shape), (C.c_double * 3)(*delta),
+                                        depth, (C.c_double * 2)(*lbound),
+                                        (C.c_double * 2)(*ubound), (C.c_int * 2)(*npoints),
+                                        out.reshape(-1), threads)
+    if rc != 0:
+        raise ValueError("min(z) is less than -h")
     return out
 
 

@@ -1,0 +1,197 @@
+"""Every benchmark configuration at its full size, through the C ABI,
+against the FP64 oracle.
+
+The oracle would take minutes on a whole surface, so each check runs the
+configuration's full GPU path (noise, recurrence, post-rescale, potential)
+and compares what the oracle can afford on the same seeds:
+
+* the leading slices of the surface before the post-rescale (the recurrence
+  is causal in t, so they depend on the leading slices of noise only);
+* the potential of the first time slices, and for the FP64 configuration the
+  potential of two late time slices computed by the oracle from the GPU's own
+  surface (a slice's potential reads slices t-1, t, t+1 only);
+* the post-rescale factor against the oracle's GSL-variance restatement on
+  the GPU's own surface.
+
+Bars: FP64 within 1e-12 (surface) and 1e-10 (potential) of the sample's
+largest magnitude; FP32 (compensated accumulation, FP64 setup) RMS <= 1e-5
+and max <= 1e-4 relative to the FP64 oracle, the north-star precision
+target, on the leading slices where both are finite (the propagating model
+grows without bound and overflows FP32 at 512 x 512; the count of finite
+slices is printed and must cover the potential sample).
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import arma_amd
+from arma_amd import _abi
+import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+TWO_PI = 2 * math.pi
+DELTA = (TWO_PI,) * 3
+
+
+def model(kind, velocity, beta):
+    acf = O.acf(kind, (10, 10, 10), velocity=velocity, beta=beta)
+    phi, _ = O.yw_choi(acf, 7)
+    return acf, phi, O.white_noise_variance(acf, phi)
+
+
+def rel_errors(got, ref):
+    """(max, rms) error relative to the reference's max and RMS magnitude on
+    the leading slices where both are finite, and the number of those slices."""
+    got = np.asarray(got, np.float64)
+    ref = np.asarray(ref, np.float64)
+    fin = (np.isfinite(got) & np.isfinite(ref)).reshape(got.shape[0], -1).all(axis=1)
+    k = len(fin) if fin.all() else int(np.argmin(fin))
+    g, r = got[:k], ref[:k]
+    d = g - r
+    return (float(np.max(np.abs(d)) / np.max(np.abs(r))),
+            float(np.sqrt(np.mean(d * d) / np.mean(r * r))), k)
+
+
+def gpu_surface(ctx, shape, phi, varwn, part, seed, precision):
+    n = math.prod(math.ceil(shape[d] / part[d]) for d in range(3))
+    params = _abi.dcmt_p521_params(n)
+    desc = ctx.ar_desc(shape, phi, varwn, params, seeds=np.full(n, seed, np.uint32),
+                       noise_mode=arma_amd.NOISE_PARTITIONED, partition=part,
+                       precision=precision, algorithm=arma_amd.AR_WAVEFRONT)
+    dt = torch.float64 if precision == arma_amd.FP64 else torch.float32
+    z = torch.empty(shape, dtype=dt, device=DEV)
+    ctx.ar_generate(desc, z)
+    return z, params
+
+
+def oracle_sample(params, seed, ts, nx, ny, part, phi, varwn):
+    sample = (ts, nx, ny)
+    n = math.prod(math.ceil(sample[d] / part[d]) for d in range(3))
+    plist = [O.MtParams(*params[i].as_tuple()) for i in range(n)]
+    eps = O.noise_partitioned(plist, np.full(n, seed, np.uint32), sample, part, math.sqrt(varwn))
+    return O.ar_generate(eps, phi)
+
+
+def run_config(ctx, shape, kind, part, seed, nz, precision, ts, tp, late=()):
+    nt, nx, ny = shape
+    fp64 = precision == arma_amd.FP64
+    acf, phi, varwn = model(kind, 0.8, (-0.8, 0.0) if kind == O.PROPAGATING else (0.8, 0.8))
+    # the GPU's FP64 Choi solve gives the same coefficients as the oracle's
+    gphi, _ = ctx.yule_walker_choi(acf, 7)
+    assert gphi.shape == phi.shape
+    np.testing.assert_allclose(gphi, phi, rtol=1e-9, atol=1e-10)
+    print(f"config {shape} {'fp64' if fp64 else 'fp32'}: phi {phi.shape}, varwn {varwn:.6g}",
+          flush=True)
+    z, params = gpu_surface(ctx, shape, phi, varwn, part, seed, precision)
+    ref = oracle_sample(params, seed, ts, nx, ny, part, phi, varwn)
+    got = z[:ts].double().cpu().numpy()
+    smax, srms, sfin = rel_errors(got, ref)
+    print(f"  surface sample {ts} slices: max {smax:.3e} rms {srms:.3e}, finite {sfin}",
+          flush=True)
+    # the potential on the whole surface, at the configuration's levels
+    # (before the post-rescale, a scale factor the potential is linear in)
+    dt = z.dtype
+    out = torch.empty((nt, nz, nx, ny), dtype=dt, device=DEV)
+    pd = ctx.pot_desc(shape, DELTA, 12.0, (0.0, -10.0), (float(nt - 1), 0.0), (nt, nz), precision)
+    ctx.linear_potential(pd, z, out)
+    gpot = out[:tp].double().cpu().numpy()
+    pref = O.linear_potential(ref[:tp + 1], DELTA, 12.0, (0.0, -10.0), (float(tp - 1), 0.0),
+                              (tp, nz))
+    pmax, prms, pfin = rel_errors(gpot, pref)
+    print(f"  potential {tp} slices x {nz} levels: max {pmax:.3e} rms {prms:.3e}, "
+          f"finite {pfin}", flush=True)
+    late_err = []
+    for t in late:
+        # the oracle's potential of late slice t from the GPU's surface
+        # (slices t-1 .. t+1 are all it reads)
+        zz = np.zeros((t + 2, nx, ny))
+        zz[t - 1:t + 2] = z[t - 1:t + 2].double().cpu().numpy()
+        lref = O.linear_potential(zz, DELTA, 12.0, (float(t), -10.0), (float(t), 0.0), (1, nz))
+        lgot = out[t:t + 1].double().cpu().numpy()
+        late_err.append(float(np.max(np.abs(lgot - lref)) / np.max(np.abs(lref))))
+        print(f"  late slice {t}: potential max {late_err[-1]:.3e}", flush=True)
+    del out
+    # the post-rescale factor on the GPU's own surface
+    if fp64:
+        host = z.cpu().numpy()
+        fref = O.rescale_factor(host, float(acf[0, 0, 0]))
+        f = ctx.rescale(z, shape, precision, float(acf[0, 0, 0]))
+        print(f"  rescale factor {f:.6e} (oracle {fref:.6e})", flush=True)
+        assert f == pytest.approx(fref, rel=1e-14, abs=1e-300)
+        np.testing.assert_array_equal(z[:2].cpu().numpy(), host[:2] * f)
+        del host
+    del z
+    torch.cuda.empty_cache()
+    return (smax, srms, sfin), (pmax, prms, pfin), late_err
+
+
+def test_config_plain_ar_128x64x64_driver(tmp_path):
+    """The CPU-runnable configuration (standing wave, 7^3, 128x64x64, MT19937
+    seed 0, sequential noise) through the arma-hip drop-in driver."""
+    import subprocess
+    import os
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "arma_amd", "host", "arma-hip")
+    arma_amd.write_mt_config(str(tmp_path / "mt.dat"),
+                             arma_amd._abi.params_array([arma_amd.mt19937_params()]))
+    (tmp_path / "c0.arma").write_text("""model = AR {
+	out_grid = (128,64,64)
+	acf = {
+		func = standing_wave
+		grid = (10,10,10)
+		amplitude = 1
+		velocity = 0.8
+		alpha = (0.06,0.06,0.06)
+		beta = (0.8,0.8)
+	}
+	order = (7,7,7)
+	algorithm = choi_recursive
+	no_seed = 1
+	validate = 0
+	output = surface,binary
+}
+velocity_potential_solver = linear {
+	depth = 12
+	domain = from (0,-10) to (127,0) npoints (128,4)
+}
+""")
+    r = subprocess.run([exe, str(tmp_path / "c0.arma")], cwd=str(tmp_path), capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    zeta = np.fromfile(tmp_path / "zeta.bin", dtype=">f8").reshape(128, 64, 64)
+    acf, phi, varwn = model(O.STANDING, 0.8, (0.8, 0.8))
+    eps = O.noise_sequential(O.mt19937_params(), 0, math.sqrt(varwn), 128 * 64 * 64)
+    ref = O.ar_generate(eps.reshape(128, 64, 64), phi)
+    ref *= O.rescale_factor(ref, float(acf[0, 0, 0]))
+    err = float(np.max(np.abs(zeta - ref)) / np.max(np.abs(ref)))
+    print(f"c0 128x64x64 through arma-hip: max rel {err:.3e}", flush=True)
+    assert err <= 1e-10
+
+
+def test_config_ar_potential_1024x256x256_fp64(hip_ctx):
+    """AR + linear potential, FP64: propagating ACF, 1024x256x256, dcmt
+    streams per 16x32x32 partition, seed 0, 16 levels on 256 x 256."""
+    (smax, _, sfin), (pmax, _, pfin), late = run_config(
+        hip_ctx, (1024, 256, 256), O.PROPAGATING, (16, 32, 32), 0, 16, arma_amd.FP64,
+        ts=32, tp=4, late=(700, 1000))
+    assert sfin == 32 and pfin == 4
+    assert smax <= 1e-12
+    assert pmax <= 1e-10
+    assert max(late) <= 1e-10
+
+
+@pytest.mark.parametrize("kind", [O.PROPAGATING, O.STANDING])
+def test_config_precision_2048x512x512_fp32(hip_ctx, kind):
+    """Precision configuration: 2048x512x512 FP32 with compensated sums and
+    FP64 setup, dcmt streams per 32x64x64 partition, seed 1, potential at 32
+    levels on 512 x 512 in FP32 with FP64 twiddles; the propagating ACF as
+    configured and the standing-wave ACF (a stable process of the same
+    order)."""
+    (smax, srms, sfin), (pmax, prms, pfin), _ = run_config(
+        hip_ctx, (2048, 512, 512), kind, (32, 64, 64), 1, 32, arma_amd.FP32, ts=64, tp=4)
+    assert pfin == 4 and sfin >= 5
+    assert srms <= 1e-5 and smax <= 1e-4
+    assert prms <= 1e-5 and pmax <= 1e-4

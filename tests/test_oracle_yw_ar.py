@@ -241,3 +241,25 @@ def test_rescale_reads_contiguous_run():
     # and it is NOT the sub-block variance
     sub = z[2:, 3:, 3:].reshape(-1)
     assert abs(np.var(sub, ddof=1) - np.var(run, ddof=1)) > 1e-6
+
+
+def test_openmp_variants_equal_sequential():
+    """The all-core CPU baseline's OpenMP entry points give the sequential
+    oracle's results bit for bit (partitions and levels are independent)."""
+    from arma_amd import _abi
+    params = [O.MtParams(*p.as_tuple()) for p in _abi.dcmt_p521_params(64)]
+    shape, part = (20, 24, 40), (8, 8, 16)
+    n = 3 * 3 * 3
+    seeds = np.arange(n, dtype=np.uint32) + 5
+    a = O.noise_partitioned(params[:n], seeds, shape, part, 0.7)
+    b = O.noise_partitioned(params[:n], seeds, shape, part, 0.7, threads=4)
+    np.testing.assert_array_equal(a, b)
+    acf = O.acf(O.STANDING, (10, 10, 10), velocity=0.8, beta=(0.8, 0.8))
+    phi, _ = O.yw_choi(acf, 5)
+    np.testing.assert_array_equal(O.ar_generate_dag(a, phi, part),
+                                  O.ar_generate_dag(a, phi, part, threads=4))
+    np.testing.assert_array_equal(O.ar_generate_dag(a, phi, part), O.ar_generate(a, phi))
+    z = O.ar_generate(a, phi)[:, :16, :32]
+    args = ((2 * np.pi,) * 3, 12.0, (0.0, -10.0), (5.0, 0.0), (6, 5))
+    np.testing.assert_array_equal(O.linear_potential(z, *args),
+                                  O.linear_potential_omp(z, *args, 4))
