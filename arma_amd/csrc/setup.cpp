@@ -711,26 +711,33 @@ int arma_hip_yule_walker_gauss(const double* acf, const int32_t acf_shape[3],
 			A(r - 1, c - 1) = gam(ri - ci, rj - cj, rk - ck);
 		}
 	}
-	// Cholesky factorisation (the block-Toeplitz AC matrix is SPD for a valid ACF)
-	for (int j = 0; j < m; ++j) {
-		double d = A(j, j);
-		for (int k = 0; k < j; ++k) d -= A(j, k) * A(j, k);
-		if (!(d > 0.0)) return ARMA_HIP_ERUNTIME;
-		const double ljj = std::sqrt(d);
-		A(j, j) = ljj;
-		for (int i = j + 1; i < m; ++i) {
-			double s = A(i, j);
-			for (int k = 0; k < j; ++k) s -= A(i, k) * A(j, k);
-			A(i, j) = s / ljj;
+	// The block-Toeplitz matrix is symmetric but not always definite (the
+	// |lag| symmetrisation of a propagating ACF makes it indefinite), so it
+	// is solved as the reference's dsysv does for any nonsingular symmetric
+	// matrix, here by Gaussian elimination with partial pivoting.
+	for (int k = 0; k < m; ++k) {
+		int p = k;
+		for (int i = k + 1; i < m; ++i)
+			if (std::fabs(A(i, k)) > std::fabs(A(p, k))) p = i;
+		if (A(p, k) == 0.0) {
+			arma_hip::set_last_error("singular Yule-Walker system (dsysv info > 0)");
+			return ARMA_HIP_ERUNTIME;
+		}
+		if (p != k) {
+			for (int j = 0; j < m; ++j) std::swap(A(k, j), A(p, j));
+			std::swap(b[k], b[p]);
+		}
+		for (int i = k + 1; i < m; ++i) {
+			const double f = A(i, k) / A(k, k);
+			if (f == 0.0) continue;
+			for (int j = k + 1; j < m; ++j) A(i, j) -= f * A(k, j);
+			b[i] -= f * b[k];
 		}
 	}
-	for (int i = 0; i < m; ++i) {
-		for (int k = 0; k < i; ++k) b[i] -= A(i, k) * b[k];
-		b[i] /= A(i, i);
-	}
 	for (int i = m - 1; i >= 0; --i) {
-		for (int k = i + 1; k < m; ++k) b[i] -= A(k, i) * b[k];
-		b[i] /= A(i, i);
+		double s = b[i];
+		for (int j = i + 1; j < m; ++j) s -= A(i, j) * b[j];
+		b[i] = s / A(i, i);
 	}
 	for (int i = 0; i < m; ++i) phi_out[i + 1] = b[i];
 	return ARMA_HIP_OK;

@@ -129,8 +129,10 @@ def test_end_to_end_surface_and_potential(tmp_path, solver):
     delta = (2 * np.pi,) * 3
     pref = O.linear_potential(zeta, delta, 12.0, (0.0, -10.0), (39.0, 0.0), (40, 4),
                               solver=solver)
-    perr = float(np.max(np.abs(pot - pref)) / np.max(np.abs(pref)))
-    assert perr < 1e-9, perr
+    # phi.csv holds 6 significant digits (the stream default the reference
+    # writes with): every value within half a unit of its 6th digit, plus the
+    # GPU's 1e-12-of-max distance from the oracle
+    assert np.all(np.abs(pot - pref) <= 5e-6 * np.abs(pref) + 1e-12 * np.abs(pref).max())
 
 
 def test_nit_input_parses(tmp_path):
