@@ -45,7 +45,7 @@ EXPORTS = [
     "arma_hip_server_release", "arma_hip_server_stats", "arma_hip_server_copy",
     "arma_hip_yule_walker_choi_device", "arma_hip_white_noise_variance_device",
     "arma_hip_validate_process", "arma_hip_nit_realisation", "arma_hip_acf_generate", "arma_hip_writer_open", "arma_hip_writer_append",
-    "arma_hip_writer_close", "arma_hip_acf_generate_device",
+    "arma_hip_writer_close", "arma_hip_ar_generate_noalloc", "arma_hip_acf_generate_device",
     "arma_hip_yule_walker_choi_device_many",
 ]
 
@@ -225,6 +225,7 @@ def lib():
         L.arma_hip_white_noise.argtypes = [vp, C.POINTER(ArDesc), vp]
         L.arma_hip_ar_recurrence.argtypes = [vp, C.POINTER(ArDesc), vp]
         L.arma_hip_ar_generate.argtypes = [vp, C.POINTER(ArDesc), vp, C.c_int]
+        L.arma_hip_ar_generate_noalloc.argtypes = [vp, C.POINTER(ArDesc), vp]
         L.arma_hip_rescale.argtypes = [vp, vp, C.POINTER(C.c_int32), C.c_int32, C.c_double,
                                        C.POINTER(C.c_double)]
         L.arma_hip_linear_potential.argtypes = [vp, C.POINTER(PotDesc), vp, vp, C.c_int, vp]
@@ -536,6 +537,11 @@ class Context:
     def ar_generate(self, desc, zeta, on_device=True):
         check(lib().arma_hip_ar_generate(self.h, C.byref(desc), C.c_void_p(_ptr(zeta)),
                                          1 if on_device else 0))
+
+    def ar_generate_noalloc(self, desc, zeta):
+        """arma_hip_ar_generate_noalloc: zeta is a device tensor; the working
+        buffers come from the library's static device scratch."""
+        check(lib().arma_hip_ar_generate_noalloc(self.h, C.byref(desc), C.c_void_p(_ptr(zeta))))
 
     def rescale(self, zeta, shape, precision, gamma0):
         f = C.c_double()

@@ -144,6 +144,7 @@ int arma_hip_init(int device, arma_hip_ctx** out) {
 		ARMA_HIP_CHECK(hipGetDeviceCount(&n));
 		ARMA_HIP_REQUIRE(device >= 0 && device < n, ARMA_HIP_EDEVICE, "no such HIP device");
 		ARMA_HIP_CHECK(hipSetDevice(device));
+		arma_hip::noalloc_setup(device);
 		auto* ctx = new arma_hip_ctx;
 		ctx->device = device;
 		hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);

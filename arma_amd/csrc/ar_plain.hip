@@ -73,11 +73,11 @@ void launch_ar_plain(arma_hip_ctx* ctx, const arma_ar_desc* d, void* zeta) {
 			}
 	}
 	offset[nlevels] = int(list.size());
-	int* dlist = (int*)ctx->flags.get(sizeof(int) * list.size());
+	int* dlist = (int*)work_buffer(ctx, ctx->flags, sizeof(int) * list.size());
 	upload(ctx, dlist, list.data(), sizeof(int) * list.size());
 	const size_t nphi = size_t(ps.x) * ps.y * ps.z;
 	const bool fp32 = d->precision == ARMA_FP32;
-	void* dphi = ctx->phi.get(nphi * 8);
+	void* dphi = work_buffer(ctx, ctx->phi, nphi * 8);
 	if (fp32) {
 		std::vector<float> pf(d->phi, d->phi + nphi);
 		upload(ctx, dphi, pf.data(), nphi * 4);

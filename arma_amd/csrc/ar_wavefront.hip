@@ -534,7 +534,7 @@ void launch_typed(arma_hip_ctx* ctx, const arma_ar_desc* d, T* zeta) {
 	const size_t n0 = append_tables(d->phi, P0, P1, P2, P0, kTY + P2 - 1, ph, offs);
 	const size_t phi_bytes = (ph.size() * sizeof(T) + 255) / 256 * 256;
 	unsigned char* dbuf =
-		(unsigned char*)ctx->phi.get(phi_bytes + offs.size() * sizeof(int) + 4);
+		(unsigned char*)work_buffer(ctx, ctx->phi, phi_bytes + offs.size() * sizeof(int) + 4);
 	T* dphi = (T*)dbuf;
 	int* doffs = (int*)(dbuf + phi_bytes);
 	upload(ctx, dphi, ph.data(), ph.size() * sizeof(T));
@@ -546,14 +546,15 @@ void launch_typed(arma_hip_ctx* ctx, const arma_ar_desc* d, T* zeta) {
 	// and a per-tile sink for out-of-range stores
 	const size_t prog_bytes = sizeof(int) * kProgStride * size_t(ntiles);
 	const size_t zero_bytes = 256;
-	unsigned char* fb = (unsigned char*)ctx->flags.get(prog_bytes + zero_bytes + size_t(ntiles) * kThreads * sizeof(T));
+	unsigned char* fb = (unsigned char*)work_buffer(
+		ctx, ctx->flags, prog_bytes + zero_bytes + size_t(ntiles) * kThreads * sizeof(T));
 	int* prog = (int*)fb;
 	const T* zero = (const T*)(fb + prog_bytes);
 	T* sink = (T*)(fb + prog_bytes + zero_bytes);
 	// mailboxes: one row of boundary values per tile and step
 	const int NB = std::max(1, mbox_width<kTX>(P1 - 1, P2 - 1));
 	const int total_u = total_steps + (P1 - 1) + (P2 - 1);
-	T* mbox = (T*)ctx->mbox.get(sizeof(T) * size_t(ntiles) * total_u * NB);
+	T* mbox = (T*)work_buffer(ctx, ctx->mbox, sizeof(T) * size_t(ntiles) * total_u * NB);
 	const int nclear = int((prog_bytes + zero_bytes) / sizeof(int));
 	reset_progress<<<div_ceil(nclear, 256), 256, 0, ctx->stream>>>(prog, nclear);
 	ARMA_HIP_CHECK(hipGetLastError());
@@ -797,6 +798,23 @@ void launch_ar_wavefront(arma_hip_ctx* ctx, const arma_ar_desc* d, void* zeta) {
 		launch_typed<float, true>(ctx, d, (float*)zeta);
 	else
 		launch_typed<double, false>(ctx, d, (double*)zeta);
+}
+
+size_t ar_wavefront_work_bytes(const arma_ar_desc* d) {
+	// the buffers launch_typed takes, in its order, each 256-byte aligned
+	const size_t es = d->precision == ARMA_FP32 ? 4 : 8;
+	const int nt = d->shape[0], nx = d->shape[1], ny = d->shape[2];
+	const int P0 = d->phi_shape[0], P1 = d->phi_shape[1], P2 = d->phi_shape[2];
+	constexpr int kTX = 16, kTY = kThreads / kTX;
+	auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+	const size_t nch = size_t(tap_chunks(P1 * P2));
+	const size_t nph = (size_t((P0 + 15) / 16 * 16) + nch * P0 * kCH + 15) / 16 * 16;
+	const size_t tables = al(nph * es) + nch * kCH * sizeof(int) + 4;
+	const size_t ntiles = size_t(div_ceil(nx, kTX)) * div_ceil(ny, kTY);
+	const size_t flags = sizeof(int) * kProgStride * ntiles + 256 + ntiles * kThreads * es;
+	const size_t NB = size_t(std::max(1, mbox_width<kTX>(P1 - 1, P2 - 1)));
+	const size_t total_u = size_t(nt) + kTX + kTY - 2 + (P1 - 1) + (P2 - 1);
+	return al(tables) + al(flags) + es * ntiles * total_u * NB;
 }
 
 bool ar_batch_fits(const std::vector<BatchMember>& m, int tile_x) {

@@ -108,6 +108,12 @@ struct arma_hip_ctx {
 	size_t writer_bytes = 0;
 	bool writer_cache_busy = false;
 	std::mutex writer_mutex;
+	// While set, work_buffer() carves device buffers from this caller-given
+	// region (bump order, 256-byte aligned) instead of growing the scratch
+	// areas above: the no-allocation entry point installs the library's
+	// static device scratch here for the duration of one call.
+	unsigned char* borrow_base = nullptr;
+	size_t borrow_bytes = 0, borrow_used = 0;
 };
 
 namespace arma_hip {
@@ -138,6 +144,19 @@ struct KernelTimer {
 	}
 };
 
+// A device buffer of n bytes for one launch sequence: the context's scratch
+// area s (grown on demand), or, while a borrowed region is installed on the
+// context, the next n bytes of that region (ARMA_HIP_EINVAL when it is full).
+inline void* work_buffer(arma_hip_ctx* ctx, Scratch& s, size_t n) {
+	if (!ctx->borrow_base) return s.get(n);
+	const size_t at = (ctx->borrow_used + 255) / 256 * 256;
+	ARMA_HIP_REQUIRE(at <= ctx->borrow_bytes && n <= ctx->borrow_bytes - at, ARMA_HIP_EINVAL,
+	                 "the request needs more device scratch than the no-allocation entry "
+	                 "point has (" + std::to_string(ctx->borrow_bytes) + " bytes)");
+	ctx->borrow_used = at + n;
+	return ctx->borrow_base + at;
+}
+
 // Host-to-device copy of a small, caller-owned host buffer, complete on return.
 inline void upload(arma_hip_ctx* ctx, void* dst, const void* src, size_t bytes) {
 	if (bytes == 0) return;
@@ -156,6 +175,14 @@ void launch_ar_plain(arma_hip_ctx* ctx, const arma_ar_desc* d, void* zeta);
 void launch_ar_wavefront(arma_hip_ctx* ctx, const arma_ar_desc* d, void* zeta);
 double launch_rescale(arma_hip_ctx* ctx, void* zeta, const int32_t shape[3], int precision,
                       double gamma0);
+
+// Device bytes the buffers of launch_white_noise and launch_ar_wavefront take
+// for d (each buffer 256-byte aligned, in the order the launchers take them).
+size_t white_noise_work_bytes(const arma_ar_desc* d);
+size_t ar_wavefront_work_bytes(const arma_ar_desc* d);
+// Looks up this device's copy of the static scratch of the no-allocation
+// entry point (once per device; the device must be current).
+void noalloc_setup(int device);
 
 // FP64 Yule-Walker (Choi) and white-noise variance on the GPU.  phi gets
 // the l x l x l coefficients before chopping.

@@ -850,11 +850,11 @@ void launch_white_noise(arma_hip_ctx* ctx, const arma_ar_desc* d, void* eps) {
 	const int ntotal = g.np.x * g.np.y * g.np.z;
 	ARMA_HIP_REQUIRE(d->n_mts >= ntotal && d->mts, ARMA_HIP_EPRNG, "bad number of MT configs");
 	validate_params(d->mts, ntotal);
-	auto* dparams = (arma_mt_params*)ctx->params.get(sizeof(arma_mt_params) * ntotal);
+	auto* dparams = (arma_mt_params*)work_buffer(ctx, ctx->params, sizeof(arma_mt_params) * ntotal);
 	upload(ctx, dparams, d->mts, sizeof(arma_mt_params) * ntotal);
 	uint32_t* dseeds = nullptr;
 	if (d->seeds) {
-		dseeds = (uint32_t*)ctx->seeds.get(sizeof(uint32_t) * ntotal);
+		dseeds = (uint32_t*)work_buffer(ctx, ctx->seeds, sizeof(uint32_t) * ntotal);
 		upload(ctx, dseeds, d->seeds, sizeof(uint32_t) * ntotal);
 	}
 	// every stream's block must fit 32-bit positions for the fast kernels
@@ -862,6 +862,16 @@ void launch_white_noise(arma_hip_ctx* ctx, const arma_ar_desc* d, void* eps) {
 	KernelTimer timer(ctx, d->noise_mode == ARMA_NOISE_SEQUENTIAL ? "noise_sequential"
 	                                                               : "noise_partitioned");
 	launch_by_type(ctx, fp32, sem, d->mts, dparams, dseeds, eps, g, ntotal, small_blocks);
+}
+
+size_t white_noise_work_bytes(const arma_ar_desc* d) {
+	size_t n = 1;
+	if (d->noise_mode == ARMA_NOISE_PARTITIONED && d->partition[0] > 0 && d->partition[1] > 0 &&
+	    d->partition[2] > 0)
+		n = size_t(div_ceil(d->shape[0], d->partition[0])) * div_ceil(d->shape[1], d->partition[1]) *
+		    div_ceil(d->shape[2], d->partition[2]);
+	const size_t params = (sizeof(arma_mt_params) * n + 255) / 256 * 256;
+	return params + (d->seeds ? sizeof(uint32_t) * n : 0);
 }
 
 void launch_mt_raw(arma_hip_ctx* ctx, const arma_mt_params* mts, const uint32_t* seeds,

@@ -148,6 +148,20 @@ int arma_hip_ar_recurrence(arma_hip_ctx* ctx, const arma_ar_desc* desc, void* ze
 int arma_hip_ar_generate(arma_hip_ctx* ctx, const arma_ar_desc* desc, void* zeta,
                          int zeta_on_device);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+   memory (tight serving loops, harnesses that forbid hipMalloc):
+   arma_hip_ar_generate with a device pointer zeta and algorithm
+   ARMA_AR_WAVEFRONT, whose working buffers (MT records and seeds,
+   coefficient tables, tile progress counters, halo mailboxes) come from a
+   static device array of 32 MiB per GPU instead of allocations.  Returns
+   ARMA_HIP_EINVAL, before enqueuing anything, for a surface whose buffers
+   exceed that (the mailboxes grow with tiles x (nt + 30 + order) x halo
+   width).  All calls on one GPU share the array: calls that overlap on a GPU
+   (other threads or streams) race, and the caller serialises them.  The
+   small host tables are uploaded with a stream synchronisation, so a call
+   cannot be captured into a HIP graph. */
+int arma_hip_ar_generate_noalloc(arma_hip_ctx* ctx, const arma_ar_desc* desc, void* zeta);
+
 /* zeta *= sqrt(gamma0 / var(run)), where var is the sample variance of the
    contiguous run of ceil(nt/2)ceil(nx/2)ceil(ny/2) elements that starts at
    zeta(nt/2, nx/2, ny/2).  factor_out (optional, host) receives the factor. */
