@@ -98,19 +98,29 @@ __device__ DMat build_Ra0(const YwArgs& y, double* dst, int b) {
 	return DMat{dst, nc, 1};
 }
 
+// Row i of A times column j of B, summed over k in order with zero factors
+// of A skipped (the host's loop).  The skip is a select rather than a
+// branch, so the loads of later terms go out while earlier terms are summed:
+// the sum is the same to the bit.
+__device__ __forceinline__ double row_col(const double* __restrict__ ar,
+                                          const double* __restrict__ bc, int k, int n) {
+	double s = 0.0;
+#pragma unroll 8
+	for (int q = 0; q < k; ++q) {
+		const double aik = ar[q];
+		const double t = s + aik * bc[size_t(q) * n];
+		s = aik == 0.0 ? s : t;
+	}
+	return s;
+}
+
 // C = A B (SUB = false) or C = C - A B (SUB = true); C may not alias A or B.
 template <bool SUB>
 __device__ void gemm(DMat C, DMat A, DMat B) {
 	const int m = A.r, k = A.c, n = B.c;
 	for (int e = threadIdx.x; e < m * n; e += blockDim.x) {
 		const int i = e / n, j = e - (e / n) * n;
-		const double* ar = A.p + size_t(i) * k;
-		double s = 0.0;
-		for (int q = 0; q < k; ++q) {
-			const double aik = ar[q];
-			if (aik == 0.0) continue;
-			s += aik * B.p[size_t(q) * n + j];
-		}
+		const double s = row_col(A.p + size_t(i) * k, B.p + j, k, n);
 		C.p[e] = SUB ? C.p[e] - s : s;
 	}
 	__syncthreads();
@@ -121,13 +131,7 @@ __device__ void gemm_from(DMat C, DMat D, DMat A, DMat B) {
 	const int m = A.r, k = A.c, n = B.c;
 	for (int e = threadIdx.x; e < m * n; e += blockDim.x) {
 		const int i = e / n, j = e - (e / n) * n;
-		const double* ar = A.p + size_t(i) * k;
-		double s = 0.0;
-		for (int q = 0; q < k; ++q) {
-			const double aik = ar[q];
-			if (aik == 0.0) continue;
-			s += aik * B.p[size_t(q) * n + j];
-		}
+		const double s = row_col(A.p + size_t(i) * k, B.p + j, k, n);
 		C.p[e] = D.p[e] - s;
 	}
 	__syncthreads();
