@@ -88,8 +88,11 @@ Matrix inverse(const Matrix& m) {
 		for (int k = 0; k < n; ++k) std::swap(col[k], col[piv[k]]);
 		for (int i = 0; i < n; ++i)
 			for (int j = 0; j < i; ++j) col[i] -= lu(i, j) * col[j];
+		// back substitution with each row's terms in descending column order
+		// (the column-oriented order of a triangular solve, which the device
+		// solver blocks by rows)
 		for (int i = n - 1; i >= 0; --i) {
-			for (int j = i + 1; j < n; ++j) col[i] -= lu(i, j) * col[j];
+			for (int j = n - 1; j > i; --j) col[i] -= lu(i, j) * col[j];
 			col[i] /= lu(i, i);
 		}
 		for (int i = 0; i < n; ++i) inv(i, c) = col[i];

@@ -98,41 +98,129 @@ __device__ DMat build_Ra0(const YwArgs& y, double* dst, int b) {
 	return DMat{dst, nc, 1};
 }
 
-// Row i of A times column j of B, summed over k in order with zero factors
-// of A skipped (the host's loop).  The skip is a select rather than a
-// branch, so the loads of later terms go out while earlier terms are summed:
-// the sum is the same to the bit.
-__device__ __forceinline__ double row_col(const double* __restrict__ ar,
-                                          const double* __restrict__ bc, int k, int n) {
-	double s = 0.0;
-#pragma unroll 8
+// Elements (i, j0 .. j0+3) of A B by one lane (cnt of them: fewer at the
+// end of a row): element (i, j) is row i of A times column j of B, summed
+// over k in order with zero factors of A skipped (the host's loop).  One
+// load of A's factor serves the four sums, whose B terms are contiguous;
+// the skip is a select rather than a branch and the four sums are
+// independent, so the loads of later terms go out while earlier terms are
+// summed: each sum is the host's to the bit.  d(j, s) receives the product
+// of column j.
+template <class D>
+__device__ __forceinline__ void products_row4(const double* __restrict__ A,
+                                              const double* __restrict__ B, int k, int n, int i,
+                                              int j0, int cnt, D&& d) {
+	const double* ar = A + size_t(i) * k;
+	const double* bc = B + j0;
+	double s[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
 	for (int q = 0; q < k; ++q) {
 		const double aik = ar[q];
-		const double t = s + aik * bc[size_t(q) * n];
-		s = aik == 0.0 ? s : t;
+		const double* bq = bc + size_t(q) * n;
+#pragma unroll
+		for (int r = 0; r < 4; ++r) {
+			const double t = s[r] + aik * bq[r < cnt ? r : 0];
+			s[r] = aik == 0.0 ? s[r] : t;
+		}
 	}
-	return s;
+#pragma unroll
+	for (int r = 0; r < 4; ++r)
+		if (r < cnt) d(j0 + r, s[r]);
+}
+
+// Products with more than a few columns go through LDS: 64 x 64 tiles of the
+// result, k in chunks of 32, a 2 x 2 block of the tile per lane (1024
+// lanes).  Each element still sums its terms over k in order with the zero
+// factors of A skipped, so the tiling changes no bit; it cuts the L2 reads
+// of one workgroup by the tile's reuse.
+constexpr int kGT = 64, kGK = 32;
+constexpr int kPoolDoubles = kGT * (kGK + 1) + kGK * (kGT + 1);
+
+// The workgroup's one LDS buffer, shared by the product tiles and the
+// substitution's row blocks (a function-scope __shared__ array is a single
+// allocation however many callers inline it).
+__device__ __forceinline__ double* lds_pool() {
+	__shared__ double pool[kPoolDoubles];
+	return pool;
+}
+
+// the products of a tile, handed to d(i, j, s) for elements in range
+template <class D>
+__device__ void tiled_products(const double* __restrict__ A, const double* __restrict__ B,
+                               int m, int k, int n, D&& d) {
+	double(*sa)[kGK + 1] = reinterpret_cast<double(*)[kGK + 1]>(lds_pool());
+	double(*sb)[kGT + 1] = reinterpret_cast<double(*)[kGT + 1]>(lds_pool() + kGT * (kGK + 1));
+	const int tid = threadIdx.x;
+	const int tx = tid % (kGT / 2), ty = tid / (kGT / 2);  // ty < 32 for 1024 lanes
+	for (int ti = 0; ti < m; ti += kGT)
+		for (int tj = 0; tj < n; tj += kGT) {
+			double s[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+			for (int kc = 0; kc < k; kc += kGK) {
+				const int kn = min(kGK, k - kc);
+				for (int e = tid; e < kGT * kGK; e += blockDim.x) {
+					const int r = e / kGK, q = e - (e / kGK) * kGK;
+					sa[r][q] = ti + r < m && q < kn ? A[size_t(ti + r) * k + kc + q] : 0.0;
+				}
+				for (int e = tid; e < kGK * kGT; e += blockDim.x) {
+					const int q = e / kGT, c = e - (e / kGT) * kGT;
+					sb[q][c] = q < kn && tj + c < n ? B[size_t(kc + q) * n + tj + c] : 0.0;
+				}
+				__syncthreads();
+				for (int q = 0; q < kn; ++q) {
+					const double a0 = sa[2 * ty][q], a1 = sa[2 * ty + 1][q];
+					const double b0 = sb[q][2 * tx], b1 = sb[q][2 * tx + 1];
+					const double t00 = s[0][0] + a0 * b0, t01 = s[0][1] + a0 * b1;
+					const double t10 = s[1][0] + a1 * b0, t11 = s[1][1] + a1 * b1;
+					s[0][0] = a0 == 0.0 ? s[0][0] : t00;
+					s[0][1] = a0 == 0.0 ? s[0][1] : t01;
+					s[1][0] = a1 == 0.0 ? s[1][0] : t10;
+					s[1][1] = a1 == 0.0 ? s[1][1] : t11;
+				}
+				__syncthreads();
+			}
+			for (int a = 0; a < 2; ++a)
+				for (int b = 0; b < 2; ++b) {
+					const int i = ti + 2 * ty + a, j = tj + 2 * tx + b;
+					if (i < m && j < n) d(i, j, s[a][b]);
+				}
+		}
 }
 
 // C = A B (SUB = false) or C = C - A B (SUB = true); C may not alias A or B.
 template <bool SUB>
 __device__ void gemm(DMat C, DMat A, DMat B) {
-	const int m = A.r, k = A.c, n = B.c;
-	for (int e = threadIdx.x; e < m * n; e += blockDim.x) {
-		const int i = e / n, j = e - (e / n) * n;
-		const double s = row_col(A.p + size_t(i) * k, B.p + j, k, n);
-		C.p[e] = SUB ? C.p[e] - s : s;
+	const int m = A.r, k = A.c, n = B.c, nb = (n + 3) / 4;
+	if (n >= 8 && blockDim.x == 1024) {
+		tiled_products(A.p, B.p, m, k, n, [&](int i, int j, double s) {
+			double* c = C.p + size_t(i) * n + j;
+			*c = SUB ? *c - s : s;
+		});
+	} else {
+		for (int e = threadIdx.x; e < m * nb; e += blockDim.x) {
+			const int i = e / nb, j0 = 4 * (e - (e / nb) * nb);
+			products_row4(A.p, B.p, k, n, i, j0, min(4, n - j0), [&](int j, double s) {
+				double* c = C.p + size_t(i) * n + j;
+				*c = SUB ? *c - s : s;
+			});
+		}
 	}
 	__syncthreads();
 }
 
 // C = D - A B (C may alias D).
 __device__ void gemm_from(DMat C, DMat D, DMat A, DMat B) {
-	const int m = A.r, k = A.c, n = B.c;
-	for (int e = threadIdx.x; e < m * n; e += blockDim.x) {
-		const int i = e / n, j = e - (e / n) * n;
-		const double s = row_col(A.p + size_t(i) * k, B.p + j, k, n);
-		C.p[e] = D.p[e] - s;
+	const int m = A.r, k = A.c, n = B.c, nb = (n + 3) / 4;
+	if (n >= 8 && blockDim.x == 1024) {
+		tiled_products(A.p, B.p, m, k, n, [&](int i, int j, double s) {
+			C.p[size_t(i) * n + j] = D.p[size_t(i) * n + j] - s;
+		});
+	} else {
+		for (int e = threadIdx.x; e < m * nb; e += blockDim.x) {
+			const int i = e / nb, j0 = 4 * (e - (e / nb) * nb);
+			products_row4(A.p, B.p, k, n, i, j0, min(4, n - j0), [&](int j, double s) {
+				C.p[size_t(i) * n + j] = D.p[size_t(i) * n + j] - s;
+			});
+		}
 	}
 	__syncthreads();
 }
@@ -151,10 +239,133 @@ __device__ double dot(const double* a, const double* b, int n) {
 	return v;
 }
 
+// The two triangular solves of the inverse, one column of it per lane, kept
+// in place in inv: column c starts as e_c through the row interchanges
+// (tracked as the position of its one), then
+//   forward  x_i = b_i - sum_{j<i} L_ij x_j      (j ascending)
+//   backward x_i = (y_i - sum_{j>i} U_ij x_j) / U_ii  (j descending),
+// each row's terms subtracted one at a time in that order, as the host
+// solver does.  Rows go in blocks of kSubRows: the terms of a block's rows
+// with j outside the block come first in either order, so the block's rows
+// take them together (one load of x_j serves kSubRows sums in registers)
+// before the in-block terms are taken row by row.  The block's rows of
+// L / U are staged through LDS in chunks of kSubCols columns (aligned, so a
+// block's own columns share one chunk); every lane reads the same element,
+// a broadcast.
+constexpr int kSubRows = 16, kSubCols = 256;
+static_assert(kSubRows * kSubCols <= kPoolDoubles, "substitution chunk exceeds the LDS pool");
+static_assert(kSubCols % kSubRows == 0, "row blocks must not straddle column chunks");
+
+constexpr int kSubX = 4;  // x_j loads in flight
+
+// v[r] -= L / U(row r, column j) * x_j for the block's rows, in registers
+__device__ __forceinline__ void sub_column(double (&v)[kSubRows], const double* b, double x) {
+#pragma unroll
+	for (int r = 0; r < kSubRows; ++r) v[r] -= b[r] * x;
+}
+
+__device__ __forceinline__ void stage_rows(double* blk, const double* lu, int n, int I, int nb, int j0, int jn) {
+	__syncthreads();  // the previous chunk's readers are done
+	for (int e = threadIdx.x; e < kSubRows * kSubCols; e += blockDim.x) {
+		const int q = e / kSubRows, r = e - q * kSubRows;  // blk[q][r]: column-major
+		blk[e] = r < nb && q < jn ? lu[size_t(I + r) * n + j0 + q] : 0.0;
+	}
+	__syncthreads();
+}
+
+__device__ __forceinline__ void substitute(const YwArgs& y, DMat inv) {
+	const int n = inv.r;
+	const double* __restrict__ lu = y.lu;
+	double* blk = lds_pool();
+	for (int c0 = 0; c0 < n; c0 += blockDim.x) {
+		const int c = c0 + threadIdx.x;
+		const bool act = c < n;
+		double* __restrict__ col = inv.p + (act ? c : 0);  // stride n
+		int pos = c;
+		if (act)
+			for (int k = 0; k < n; ++k) {
+				const int p = y.piv[k];
+				pos = pos == k ? p : (pos == p ? k : pos);
+			}
+		// forward, blocks top down
+		for (int I = 0; I < n; I += kSubRows) {
+			const int nb = min(kSubRows, n - I);
+			double v[kSubRows];
+#pragma unroll
+			for (int r = 0; r < kSubRows; ++r) v[r] = I + r == pos ? 1.0 : 0.0;
+			for (int j0 = 0; j0 < I + nb; j0 += kSubCols) {
+				const int jn = min(kSubCols, n - j0);
+				stage_rows(blk, lu, n, I, nb, j0, jn);
+				if (!act) continue;
+				const int je = min(jn, I - j0);  // outer columns of this chunk
+				int q = 0;
+				for (; q + kSubX <= je; q += kSubX) {
+					double x[kSubX];  // the loads go out together
+#pragma unroll
+					for (int u = 0; u < kSubX; ++u) x[u] = col[size_t(j0 + q + u) * n];
+#pragma unroll
+					for (int u = 0; u < kSubX; ++u) sub_column(v, blk + (q + u) * kSubRows, x[u]);
+				}
+				for (; q < je; ++q) sub_column(v, blk + q * kSubRows, col[size_t(j0 + q) * n]);
+				if (I < j0 + jn) {  // the block's own columns: unit lower triangle
+					const double* b = blk + (I - j0) * kSubRows;
+#pragma unroll
+					for (int r = 1; r < kSubRows; ++r)
+#pragma unroll
+						for (int t = 0; t < r; ++t) v[r] -= b[t * kSubRows + r] * v[t];
+				}
+			}
+			if (act)
+#pragma unroll
+				for (int r = 0; r < kSubRows; ++r)
+					if (r < nb) col[size_t(I + r) * n] = v[r];
+		}
+		// backward, blocks bottom up; chunks from the last column down
+		for (int I = (n - 1) / kSubRows * kSubRows; I >= 0; I -= kSubRows) {
+			const int nb = min(kSubRows, n - I);
+			double v[kSubRows];
+#pragma unroll
+			for (int r = 0; r < kSubRows; ++r) v[r] = act && r < nb ? col[size_t(I + r) * n] : 0.0;
+			for (int j0 = (n - 1) / kSubCols * kSubCols; j0 >= I / kSubCols * kSubCols;
+			     j0 -= kSubCols) {
+				const int jn = min(kSubCols, n - j0);
+				stage_rows(blk, lu, n, I, nb, j0, jn);
+				if (!act) continue;
+				const int qlo = max(0, I + nb - j0);  // outer columns are j >= I + nb
+				int q = jn - 1;
+				for (; q - kSubX + 1 >= qlo; q -= kSubX) {
+					double x[kSubX];
+#pragma unroll
+					for (int u = 0; u < kSubX; ++u) x[u] = col[size_t(j0 + q - u) * n];
+#pragma unroll
+					for (int u = 0; u < kSubX; ++u) sub_column(v, blk + (q - u) * kSubRows, x[u]);
+				}
+				for (; q >= qlo; --q) sub_column(v, blk + q * kSubRows, col[size_t(j0 + q) * n]);
+				if (j0 <= I) {  // the block's own columns: upper triangle, then the pivot
+					const double* b = blk + (I - j0) * kSubRows;
+#pragma unroll
+					for (int r = kSubRows - 1; r >= 0; --r) {
+						if (r >= nb) continue;
+#pragma unroll
+						for (int t = kSubRows - 1; t > r; --t)
+							if (t < nb) v[r] -= b[t * kSubRows + r] * v[t];
+						v[r] = v[r] / b[r * kSubRows + r];
+					}
+				}
+			}
+			if (act)
+#pragma unroll
+				for (int r = 0; r < kSubRows; ++r)
+					if (r < nb) col[size_t(I + r) * n] = v[r];
+		}
+	}
+	__syncthreads();
+}
+
 // inv = src^{-1} by LU with partial pivoting and one substitution per column
 // (getrf + getri in the order of the host solver).  Returns false when a
 // pivot is exactly zero.
-__device__ bool inverse(const YwArgs& y, DMat inv, DMat src) {
+__device__ __forceinline__ bool inverse(const YwArgs& y, DMat inv, DMat src) {
 	const int n = src.r;
 	double* lu = y.lu;
 	for (int e = threadIdx.x; e < n * n; e += blockDim.x) lu[e] = src.p[e];
@@ -220,35 +431,31 @@ __device__ bool inverse(const YwArgs& y, DMat inv, DMat src) {
 		const double d = lu[size_t(k) * n + k];
 		for (int i = k + 1 + threadIdx.x; i < n; i += blockDim.x) lu[size_t(i) * n + k] /= d;
 		__syncthreads();
-		const int w = n - k - 1;
-		for (int e = threadIdx.x; e < w * w; e += blockDim.x) {
-			const int i = k + 1 + e / w, j = k + 1 + (e - (e / w) * w);
-			lu[size_t(i) * n + j] -= lu[size_t(i) * n + k] * lu[size_t(k) * n + j];
+		// the rank-1 update, four consecutive elements of a row per lane:
+		// every operand is loaded before any of the four is stored (an updated
+		// element (i > k, j > k) is never an operand of another: those are
+		// column k, row k and the element itself)
+		const int w = n - k - 1, wb = (w + 3) / 4;
+		for (int e = threadIdx.x; e < w * wb; e += blockDim.x) {
+			const int i = k + 1 + e / wb, j0 = k + 1 + 4 * (e - (e / wb) * wb);
+			const int cnt = min(4, n - j0);
+			double* row = lu + size_t(i) * n;
+			const double* rk = lu + size_t(k) * n;
+			const double l = row[k];
+			double a[4], u[4];
+#pragma unroll
+			for (int r = 0; r < 4; ++r) {
+				const int j = r < cnt ? j0 + r : j0;
+				a[r] = row[j];
+				u[r] = rk[j];
+			}
+#pragma unroll
+			for (int r = 0; r < 4; ++r)
+				if (r < cnt) row[j0 + r] = a[r] - l * u[r];
 		}
 		__syncthreads();
 	}
-	// column c of the inverse by lane c, kept in place in inv
-	for (int c = threadIdx.x; c < n; c += blockDim.x) {
-		double* col = inv.p + c;  // stride n
-		for (int i = 0; i < n; ++i) col[size_t(i) * n] = i == c ? 1.0 : 0.0;
-		for (int k = 0; k < n; ++k) {
-			const int p = y.piv[k];
-			const double t = col[size_t(k) * n];
-			col[size_t(k) * n] = col[size_t(p) * n];
-			col[size_t(p) * n] = t;
-		}
-		for (int i = 0; i < n; ++i) {
-			double v = col[size_t(i) * n];
-			for (int j = 0; j < i; ++j) v -= lu[size_t(i) * n + j] * col[size_t(j) * n];
-			col[size_t(i) * n] = v;
-		}
-		for (int i = n - 1; i >= 0; --i) {
-			double v = col[size_t(i) * n];
-			for (int j = i + 1; j < n; ++j) v -= lu[size_t(i) * n + j] * col[size_t(j) * n];
-			col[size_t(i) * n] = v / lu[size_t(i) * n + i];
-		}
-	}
-	__syncthreads();
+	substitute(y, inv);
 	return true;
 }
 
