@@ -19,8 +19,12 @@
 // the accepted ones with a workgroup prefix sum.
 //
 // All arithmetic is compiled without FP contraction, so every draw is
-// computed with the roundings the reference source writes.
+// computed with the roundings the reference source writes.  The FP64 polar
+// step's logarithm is log_dd (log_dd.hpp): correctly rounded, where the
+// device library's log is within an ulp, so a draw differs from the host's
+// only where the host's log is not correctly rounded.
 #include "common.hpp"
+#include "log_dd.hpp"
 
 #include <vector>
 
@@ -28,6 +32,12 @@ namespace arma_hip {
 namespace {
 
 constexpr int kMaxNN = 1392;  // largest dcmt state (p = 44497)
+
+// The log table staged in LDS by the whole workgroup (before any lane leaves).
+__device__ __forceinline__ void stage_log_table(double* lt) {
+	for (int i = threadIdx.x; i < kLogDdRows * 3; i += blockDim.x) lt[i] = kLogDdTable[i];
+	__syncthreads();
+}
 
 __device__ __forceinline__ uint32_t mt_temper(uint32_t x, const arma_mt_params& p) {
 	x ^= x >> p.shift0;
@@ -104,7 +114,7 @@ struct PolarD {
 	double saved = 0.0;
 
 	template <class G>
-	__device__ double next(G& g, double stddev) {
+	__device__ double next(G& g, double stddev, const double* ltab) {
 		double ret;
 		if (saved_available) {
 			saved_available = false;
@@ -116,7 +126,7 @@ struct PolarD {
 				y = 2.0 * canonical_d(g) - 1.0;
 				r2 = x * x + y * y;
 			} while (r2 > 1.0 || r2 == 0.0);
-			const double mult = sqrt(-2.0 * log(r2) / r2);
+			const double mult = sqrt(-2.0 * log_dd(r2, ltab) / r2);
 			saved = x * mult;
 			saved_available = true;
 			ret = y * mult;
@@ -223,7 +233,7 @@ __global__ void noise_partitioned_kernel(const arma_mt_params* __restrict__ para
 			OutT* row = k.out + ((size_t)(k.lo0 + t) * k.sy + (k.lo1 + x)) * k.sz + k.lo2;
 			for (int y = 0; y < k.e2; ++y) {
 				if (SEM == ARMA_NOISE_DOUBLE) {
-					row[y] = OutT(nd.next(g, k.sd));
+					row[y] = OutT(nd.next(g, k.sd, kLogDdTable));
 				} else {
 					row[y] = OutT(nf.next(g, k.sdf));
 				}
@@ -293,6 +303,8 @@ __global__ __launch_bounds__(THREADS) void noise_stream_kernel(
 	__shared__ uint32_t st[MAXNN];
 	__shared__ uint32_t buf[round_words + MAXNN];
 	__shared__ int wave_tot[THREADS / 64];
+	__shared__ double ltab[kLogDdRows * 3];
+	stage_log_table(ltab);
 	const int tid = threadIdx.x;
 	const int lane = tid & 63, wave = tid >> 6;
 	const int pid = blockIdx.x;
@@ -379,7 +391,7 @@ __global__ __launch_bounds__(THREADS) void noise_stream_kernel(
 			const long long pos = produced + 2ll * (before + below);
 			if (pos < n) {
 				if (SEM == ARMA_NOISE_DOUBLE) {
-					const double mult = sqrt(-2.0 * log(r2d) / r2d);
+					const double mult = sqrt(-2.0 * log_dd(r2d, ltab) / r2d);
 					put(pos, OutT(yd * mult * stddev + 0.0));
 					if (pos + 1 < n) put(pos + 1, OutT(xd * mult * stddev + 0.0));
 				} else {
@@ -438,6 +450,8 @@ __global__ __launch_bounds__(64) void noise_seg_kernel(
 	constexpr int RW = kSegLanes * WPA;                     // words per round
 	__shared__ uint32_t st_all[2][32];
 	__shared__ uint32_t buf_all[2][RW + 32];
+	__shared__ double ltab[kLogDdRows * 3];
+	stage_log_table(ltab);
 	const int seg = threadIdx.x >> 5;
 	const int lane = threadIdx.x & 31;
 	const int pid = blockIdx.x * 2 + seg;
@@ -542,7 +556,7 @@ __global__ __launch_bounds__(64) void noise_seg_kernel(
 			const long long pos = produced + 2ll * below;
 			if (pos < n) {
 				if (SEM == ARMA_NOISE_DOUBLE) {
-					const double mult = sqrt(-2.0 * log(r2d) / r2d);
+					const double mult = sqrt(-2.0 * log_dd(r2d, ltab) / r2d);
 					put(pos, OutT(yd * mult * stddev + 0.0));
 					if (pos + 1 < n) put(pos + 1, OutT(xd * mult * stddev + 0.0));
 				} else {
@@ -605,6 +619,8 @@ __global__ __launch_bounds__(256) void noise_wave_kernel(const arma_mt_params* _
 	constexpr int WPA = SEM == ARMA_NOISE_DOUBLE ? 4 : 2;  // words per attempt
 	constexpr int RW = 64 * WPA;                            // words per round
 	__shared__ uint32_t buf_all[4][RW + 32];
+	__shared__ double ltab[kLogDdRows * 3];
+	stage_log_table(ltab);
 	const int lane = threadIdx.x & 63;
 	const int pid = __builtin_amdgcn_readfirstlane(int(blockIdx.x) * 4 + int(threadIdx.x >> 6));
 	const bool valid = pid < nstreams;
@@ -693,7 +709,7 @@ __global__ __launch_bounds__(256) void noise_wave_kernel(const arma_mt_params* _
 			const long long pos = produced + 2ll * below;
 			if (pos < n) {
 				if (SEM == ARMA_NOISE_DOUBLE) {
-					const double mult = sqrt(-2.0 * log(r2d) / r2d);
+					const double mult = sqrt(-2.0 * log_dd(r2d, ltab) / r2d);
 					put(pos, OutT(yd * mult * stddev + 0.0));
 					if (pos + 1 < n) put(pos + 1, OutT(xd * mult * stddev + 0.0));
 				} else {

@@ -1,9 +1,10 @@
 """GPU parity of the white noise and the AR recurrence, through the C ABI.
 
-Bars: uint32 streams bit-exact; normals within 4 ulp of the oracle (the
-device log may differ from glibc's by an ulp) with the same accept/reject
-pattern; the plain FP64 recurrence bit-exact; everything else to the
-tolerance written in each test.
+Bars: uint32 streams bit-exact; normals within 4 ulp of the oracle with the
+same accept/reject pattern (the device's FP64 log is correctly rounded,
+glibc's is not always: a draw differs where glibc's log misrounds; the FP32
+path's logf differs from glibc's logf at times); the plain FP64 recurrence
+bit-exact; everything else to the tolerance written in each test.
 """
 import numpy as np
 import pytest
@@ -57,8 +58,8 @@ def test_partitioned_noise_match_rate_1024x256x256(hip_ctx):
     streams per 16x32x32 partition, seed 0: 67 M normals) against the oracle:
     the fraction of draws equal to the bit, and the largest difference in
     ulp.  The uniforms, the accept/reject decisions, sqrt and the divide are
-    exact on both sides; a draw differs only where the device log rounds
-    differently from glibc's."""
+    exact on both sides and the device's log is correctly rounded
+    (test_log_dd.py), so a draw differs only where glibc's log is not."""
     import math
     import os
     shape, part = (1024, 256, 256), (16, 32, 32)
@@ -78,9 +79,10 @@ def test_partitioned_noise_match_rate_1024x256x256(hip_ctx):
     u = float(ulp_diff(got, ref))
     print(f"noise 1024x256x256 fp64: {same * 100:.4f} % of draws bit-exact, max {u:.0f} ulp",
           flush=True)
-    # measured on MI355X: 98.84 % bit-exact, at most 3 ulp
+    # measured on MI355X: 99.964 % bit-exact, at most 3 ulp (98.84 % with
+    # the device library's log)
     assert u <= 4, u
-    assert same >= 0.988, same
+    assert same >= 0.9996, same
 
 
 def test_sequential_noise_float_semantics(hip_ctx):
