@@ -28,14 +28,7 @@
 
 namespace arma_hip {
 
-__host__ __device__ inline void two_sum(double a, double b, double& s, double& err) {
-	s = a + b;
-	const double bp = s - a;
-	const double ap = s - bp;
-	err = (a - ap) + (b - bp);
-}
-
-// |a| >= |b| or a == 0
+// s + err = a + b exactly, for |a| >= |b| or a == 0 (Dekker's Fast2Sum)
 __host__ __device__ inline void fast_two_sum(double a, double b, double& s, double& err) {
 	s = a + b;
 	err = b - (s - a);
@@ -78,8 +71,11 @@ __host__ __device__ inline double log_dd(double x, const double* tab) {
 	const double lp_lo = ae + fl - hl + t3;
 	const double ed = double(e);
 	double s1, e1, s2, e2;
-	two_sum(ed * kLn2Hi, row[0], s1, e1);
-	two_sum(s1, ah, s2, e2);
+	// |e ln2| >= 0.69 > |ln c| unless e = 0; then |ln c| >= ln(129/128) >
+	// |f| ~ |ah| unless c = 1, where ln c = 0: each sum's first operand is
+	// zero or the larger, so the fast two-sums are exact
+	fast_two_sum(ed * kLn2Hi, row[0], s1, e1);
+	fast_two_sum(s1, ah, s2, e2);
 	const double lo = e1 + e2 + ed * kLn2Lo + row[1] + lp_lo;
 	return s2 + lo;
 }

@@ -577,55 +577,102 @@ __global__ __launch_bounds__(64) void noise_seg_kernel(
 	}
 }
 
-// ---- one stream per wave, the MT521 state in registers ----
+// ---- MT521 streams in registers: producer and consumer waves ----
 //
 // For the dynamic-creator records of p = 521 with 32-bit words (nn = 17,
-// mm = 8, the records the partitioned configurations use), a wave runs one
-// stream with state word k in lane k.  A twist, in genrand_mt's sequential
-// order (src/parallel_mt.hh:236), is
+// mm = 8, the records the partitioned configurations use), each 16-lane row
+// of a producer wave runs one stream's twist: lane k of the row holds state
+// word k (k < 16) and lane 15 also holds word 16.  A twist, in genrand_mt's
+// sequential order (src/parallel_mt.hh:236), is
 //   new[k]  = tw(old[k], old[k+1], old[k+8])     k = 0..8
 //   new[k]  = tw(old[k], old[k+1], new[k-9])     k = 9..15
 //   new[16] = tw(old[16], new[0], new[7])
-// which takes three row shifts inside the wave's first 16-lane row (DPP)
-// and three lane reads for the values that cross into or out of lane 16.
-// Each round tempers twists into the wave's LDS word buffer until 64
-// attempts' worth is there, evaluates the 64 polar attempts in parallel and
-// places the accepted pairs where the sequential generator puts them.  The
-// four streams of a workgroup are independent waves: their LDS accesses
-// are ordered within the wave, with no workgroup barrier.
+// where tw(a, b, s) = s ^ y(a, b) and y depends on old words only, so a twist
+// is one y per lane, the row shifts that bring old[k+1], old[k+8], new[k-9],
+// new[0] and new[7] into place (DPP within each row, so the four streams of
+// a wave never mix), and two XORs: 68 words for about the instructions the
+// 17 words of one stream per wave took.
+//
+// The untempered words go into a 1024-word ring per stream in LDS.  A
+// consumer wave takes the same four streams: lane k of a row evaluates
+// attempts 16a + k (a < 4 in FP64 semantics, a < 8 in FP32) of its stream's
+// round of 256 words, tempering the words it reads, with the attempts'
+// chains interleaved, and stores the accepted pairs where
+// the sequential generator puts them (prefix counts over the row's 16 bits of
+// the wave ballots).  Both roles step through rounds together, one workgroup
+// barrier per round: while the consumers use round r the producers twist
+// round r + 1, so a twist's dependent chain and the attempts' transcendental
+// chains issue side by side.  A workgroup is eight waves, four stream groups,
+// with the roles ordered P0 C1 P2 C3 C0 P1 C2 P3 so that waves w and w + 4
+// (the same SIMD when waves are placed round-robin) are one producer and one
+// consumer.  The consumers publish per round whether their streams are
+// complete (flags alternating by round parity, so a flag is rewritten only
+// after every wave has read it); all waves leave after the same barrier.
 
 constexpr int kWaveNN = 17, kWaveMM = 8;
+constexpr int kRingWords = 1024;  // >= a round in use + a round + 17 written ahead
+constexpr int kRoundWords = 256;  // words per round per stream
+constexpr int kPcThreads = 512;
 
-__device__ __forceinline__ uint32_t row_shl1(uint32_t v) {
-	return uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x101, 0xf, 0xf, true));
+template <int CTRL>
+__device__ __forceinline__ uint32_t row_move(uint32_t v) {
+	return uint32_t(__builtin_amdgcn_update_dpp(0, int(v), CTRL, 0xf, 0xf, true));
 }
-__device__ __forceinline__ uint32_t row_shl8(uint32_t v) {
-	return uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x108, 0xf, 0xf, true));
+// as row_move, with lanes whose source falls outside the row keeping old
+template <int CTRL>
+__device__ __forceinline__ uint32_t row_move_or(uint32_t old, uint32_t v) {
+	return uint32_t(__builtin_amdgcn_update_dpp(int(old), int(v), CTRL, 0xf, 0xf, false));
 }
-__device__ __forceinline__ uint32_t row_shr9(uint32_t v) {
-	return uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x119, 0xf, 0xf, true));
-}
+constexpr int kRowShl1 = 0x101, kRowShl7 = 0x107, kRowShl8 = 0x108;  // lane k <- lane k + n
+constexpr int kRowShr8 = 0x118, kRowShr9 = 0x119, kRowShr15 = 0x11f;  // lane k <- lane k - n
 
-__device__ __forceinline__ void wave_lds_order() {
-	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-	__builtin_amdgcn_wave_barrier();
-	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+// y(a, b) = ((a & upper) | (b & lower)) >> 1, xor aaa when the low bit is set
+__device__ __forceinline__ uint32_t mt_twist_y(uint32_t a, uint32_t b, const arma_mt_params& p) {
+	const uint32_t x = (a & p.umask) | (b & p.lmask);
+	return (x >> 1) ^ ((x & 1u) ? p.aaa : 0u);
 }
 
 template <class OutT, int SEM, class Geo>
-__global__ __launch_bounds__(256) void noise_wave_kernel(const arma_mt_params* __restrict__ params,
-                                                         const uint32_t* __restrict__ seeds,
-                                                         Geo geo, int nstreams) {
+__global__ __launch_bounds__(kPcThreads) void noise_wave_kernel(
+	const arma_mt_params* __restrict__ params, const uint32_t* __restrict__ seeds, Geo geo,
+	int nstreams) {
 	constexpr int WPA = SEM == ARMA_NOISE_DOUBLE ? 4 : 2;  // words per attempt
-	constexpr int RW = 64 * WPA;                            // words per round
-	__shared__ uint32_t buf_all[4][RW + 32];
+	constexpr int A = kRoundWords / (16 * WPA);            // attempts per lane per round
+	static_assert(A * 16 * WPA == kRoundWords, "a round is whole attempts");
+	static_assert(3 * kRoundWords + kWaveNN <= kRingWords, "ring too small");
+	__shared__ __align__(16) uint32_t ring_all[4][4][kRingWords];  // [group][row][word]
+	__shared__ int gdone[2][4];
 	__shared__ double ltab[kLogDdRows * 3];
 	stage_log_table(ltab);
-	const int lane = threadIdx.x & 63;
-	const int pid = __builtin_amdgcn_readfirstlane(int(blockIdx.x) * 4 + int(threadIdx.x >> 6));
+	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+	const int row = lane >> 4, k = lane & 15;
+	const int g = wave & 3;
+	const bool producer = ((wave >> 2) ^ (wave & 1)) == 0;
+	const int pid = (int(blockIdx.x) * 4 + g) * 4 + row;
 	const bool valid = pid < nstreams;
-	uint32_t* buf = buf_all[threadIdx.x >> 6];
+	uint32_t* ring = ring_all[g][row];
 	const arma_mt_params p = params[valid ? pid : 0];
+
+	// producer: the state (lane k: word k; word 16 on lane 15, held by all)
+	uint32_t s = 0, t = 0;
+	int wpos = 0;
+	auto twist = [&]() {
+		const uint32_t nx1 = row_move_or<kRowShl1>(t, s);  // old[k + 1]; lane 15: old[16]
+		const uint32_t t8 = row_move<kRowShl7>(t);          // lane 8 <- old[16]
+		const uint32_t nx8 = row_move_or<kRowShl8>(t8, s);  // old[k + 8]; lane 8: old[16]
+		const uint32_t y = mt_twist_y(s, nx1, p);
+		const uint32_t a = nx8 ^ y;                  // new[k], k <= 8
+		const uint32_t a9 = row_move<kRowShr9>(a);   // new[k - 9]
+		const uint32_t a0 = row_move<kRowShr15>(a);  // lane 15 <- new[0]
+		const uint32_t a7 = row_move<kRowShr8>(a);   // lane 15 <- new[7]
+		t = a7 ^ mt_twist_y(t, a0, p);               // new[16] (lane 15)
+		s = k <= kWaveMM ? a : (a9 ^ y);
+		ring[(wpos + k) & (kRingWords - 1)] = s;
+		if (k == 15) ring[(wpos + 16) & (kRingWords - 1)] = t;
+		wpos += kWaveNN;
+	};
+
+	// consumer: the block of the grid this row's stream fills
 	const StreamBlock<OutT> blk = geo(valid ? pid : 0);
 	OutT* const out = blk.out;
 	const int sy = blk.sy, sz = blk.sz;
@@ -638,7 +685,7 @@ __global__ __launch_bounds__(256) void noise_wave_kernel(const arma_mt_params* _
 	const double r12 = 1.0 / double(e12), r2 = 1.0 / double(e2);
 	const bool pow2 = (e2 & (e2 - 1)) == 0 && (e12 & (e12 - 1)) == 0;
 	const int s2 = __ffs(unsigned(e2)) - 1, s12 = __ffsll((unsigned long long)e12) - 1;
-	auto put = [&](long long q, OutT v) {
+	auto index = [&](long long q) {
 		// q < 2^31: 32-bit quotients
 		const uint32_t uq = uint32_t(q), u12 = uint32_t(e12), u2 = uint32_t(e2);
 		uint32_t qt, rr, qx;
@@ -655,77 +702,92 @@ __global__ __launch_bounds__(256) void noise_wave_kernel(const arma_mt_params* _
 			if (uint64_t(qx) * u2 > rr) --qx;
 			else if (uint64_t(qx + 1) * u2 <= rr) ++qx;
 		}
-		out[(size_t(lo0 + qt) * sy + (lo1 + qx)) * sz + (lo2 + (rr - qx * u2))] = v;
+		return (size_t(lo0 + qt) * sy + (lo1 + qx)) * sz + (lo2 + (rr - qx * u2));
 	};
-	// sgenrand_mt: word k of the recurrence from the seed (lane k keeps its own)
-	uint32_t s = 0;
-	{
-		uint32_t x = valid && seeds ? seeds[pid] : 0u;
-		for (int k = 0; k < kWaveNN; ++k) {
-			if (k == lane) s = x & p.wmask;
-			x = 1812433253u * (x ^ (x >> 30)) + uint32_t(k + 1);
-		}
-	}
-	int avail = 0;
+	// the pair at positions q (even) and q + 1: with an even row length the
+	// second is the next element of the same row
+	const bool even_rows = (e2 & 1) == 0;
+	auto put_pair = [&](long long q, OutT v0, OutT v1) {
+		const size_t i0 = index(q);
+		out[i0] = v0;
+		if (q + 1 < n) out[even_rows ? i0 + 1 : index(q + 1)] = v1;
+	};
 	long long produced = 0;
-	while (produced < n) {
-		while (avail < RW) {
-			// one twist of the 17 words (every lane takes part in the shifts)
-			const uint32_t s16 = uint32_t(__builtin_amdgcn_readlane(int(s), 16));
-			uint32_t nx1 = row_shl1(s), nx8 = row_shl8(s);
-			if (lane == 15) nx1 = s16;
-			if (lane == 8) nx8 = s16;
-			const uint32_t a = mt_twist_word(s, nx1, nx8, p);  // lanes 0..8
-			const uint32_t a9 = row_shr9(a);                   // lane k >= 9: a[k - 9]
-			const uint32_t n0 = uint32_t(__builtin_amdgcn_readlane(int(a), 0));
-			const uint32_t n7 = uint32_t(__builtin_amdgcn_readlane(int(a), kWaveMM - 1));
-			const uint32_t b = mt_twist_word(s, nx1, a9, p);  // lanes 9..15
-			const uint32_t c = mt_twist_word(s, n0, n7, p);   // lane 16
-			s = lane < kWaveNN - kWaveMM ? a : (lane < kWaveNN - 1 ? b : c);
-			if (lane < kWaveNN) buf[avail + lane] = mt_temper(s, p);
-			avail += kWaveNN;
+	auto consume = [&](int r) {
+		// attempts a * 16 + k (a < A) of the row's stream in round r; the
+		// transform is evaluated on every lane (a rejected attempt on a
+		// stand-in radius) and only accepted pairs are stored
+		const uint32_t* words = ring + ((r * kRoundWords) & (kRingWords - 1));
+		bool acc[A];
+		double xd[A], yd[A], r2d[A];
+		float xf[A], yf[A], r2f[A];
+#pragma unroll
+		for (int a = 0; a < A; ++a) {
+			const uint32_t* wv = words + (a * 16 + k) * WPA;
+			if (SEM == ARMA_NOISE_DOUBLE) {
+				const uint4 w = *reinterpret_cast<const uint4*>(wv);
+				xd[a] = 2.0 * canonical_from(mt_temper(w.x, p), mt_temper(w.y, p)) - 1.0;
+				yd[a] = 2.0 * canonical_from(mt_temper(w.z, p), mt_temper(w.w, p)) - 1.0;
+				r2d[a] = xd[a] * xd[a] + yd[a] * yd[a];
+				acc[a] = !(r2d[a] > 1.0 || r2d[a] == 0.0);
+			} else {
+				const uint2 w = *reinterpret_cast<const uint2*>(wv);
+				xf[a] = 2.0f * canonical_from_f(mt_temper(w.x, p)) - 1.0f;
+				yf[a] = 2.0f * canonical_from_f(mt_temper(w.y, p)) - 1.0f;
+				r2f[a] = xf[a] * xf[a] + yf[a] * yf[a];
+				acc[a] = !(r2f[a] > 1.0f || r2f[a] == 0.0f);
+			}
 		}
-		wave_lds_order();
-		// one attempt per lane
-		const uint32_t* wv = buf + lane * WPA;
-		bool accept;
-		double xd = 0, yd = 0, r2d = 0;
-		float xf = 0, yf = 0, r2f = 0;
-		if (SEM == ARMA_NOISE_DOUBLE) {
-			xd = 2.0 * canonical_from(wv[0], wv[1]) - 1.0;
-			yd = 2.0 * canonical_from(wv[2], wv[3]) - 1.0;
-			r2d = xd * xd + yd * yd;
-			accept = !(r2d > 1.0 || r2d == 0.0);
-		} else {
-			xf = 2.0f * canonical_from_f(wv[0]) - 1.0f;
-			yf = 2.0f * canonical_from_f(wv[1]) - 1.0f;
-			r2f = xf * xf + yf * yf;
-			accept = !(r2f > 1.0f || r2f == 0.0f);
+		int below[A];
+		int total = 0;
+#pragma unroll
+		for (int a = 0; a < A; ++a) {
+			const uint32_t bits = uint32_t(__ballot(acc[a]) >> (16 * row)) & 0xffffu;
+			below[a] = total + __popc(bits & ((1u << k) - 1u));
+			total += __popc(bits);
 		}
-		const unsigned long long mask = __ballot(accept);
-		const int below = __popcll(mask & ((1ull << lane) - 1ull));
-		const int total = __popcll(mask);
-		if (accept) {
-			const long long pos = produced + 2ll * below;
-			if (pos < n) {
-				if (SEM == ARMA_NOISE_DOUBLE) {
-					const double mult = sqrt(-2.0 * log_dd(r2d, ltab) / r2d);
-					put(pos, OutT(yd * mult * stddev + 0.0));
-					if (pos + 1 < n) put(pos + 1, OutT(xd * mult * stddev + 0.0));
-				} else {
-					const float mult = sqrtf(-2.0f * logf(r2f) / r2f);
-					put(pos, OutT(yf * mult * stddev_f + 0.0f));
-					if (pos + 1 < n) put(pos + 1, OutT(xf * mult * stddev_f + 0.0f));
-				}
+#pragma unroll
+		for (int a = 0; a < A; ++a) {
+			const long long pos = produced + 2ll * below[a];
+			if (SEM == ARMA_NOISE_DOUBLE) {
+				const double rr2 = acc[a] ? r2d[a] : 0.5;
+				const double mult = sqrt(-2.0 * log_dd(rr2, ltab) / rr2);
+				if (acc[a] && pos < n)
+					put_pair(pos, OutT(yd[a] * mult * stddev + 0.0), OutT(xd[a] * mult * stddev + 0.0));
+			} else {
+				const float rr2 = acc[a] ? r2f[a] : 0.5f;
+				const float mult = sqrtf(-2.0f * logf(rr2) / rr2);
+				if (acc[a] && pos < n)
+					put_pair(pos, OutT(yf[a] * mult * stddev_f + 0.0f),
+					         OutT(xf[a] * mult * stddev_f + 0.0f));
 			}
 		}
 		produced += 2ll * total;
-		// keep the words past the round (fewer than 17)
-		const int left = avail - RW;
-		const uint32_t keep = lane < left ? buf[RW + lane] : 0u;
-		wave_lds_order();
-		if (lane < left) buf[lane] = keep;
-		avail = left;
+	};
+
+	if (producer) {
+		// sgenrand_mt: word w of the recurrence from the seed; lane k keeps
+		// word k, every lane keeps word 16 (lane 15's copy is the one used)
+		uint32_t x = valid && seeds ? seeds[pid] : 0u;
+		for (int w = 0; w < kWaveNN; ++w) {
+			const uint32_t v = x & p.wmask;
+			if (w == k) s = v;
+			if (w == kWaveNN - 1) t = v;
+			x = 1812433253u * (x ^ (x >> 30)) + uint32_t(w + 1);
+		}
+		while (wpos < kRoundWords) twist();
+	}
+	__syncthreads();  // round 0 is in the rings
+	for (int r = 0;; ++r) {
+		if (producer) {
+			while (wpos < (r + 2) * kRoundWords) twist();  // round r + 1
+		} else {
+			consume(r);
+			const bool mine_done = !__any(produced < n);
+			if (lane == 0) gdone[r & 1][g] = mine_done;
+		}
+		__syncthreads();
+		if (gdone[r & 1][0] && gdone[r & 1][1] && gdone[r & 1][2] && gdone[r & 1][3]) break;
 	}
 }
 
@@ -777,7 +839,8 @@ void launch_streams(arma_hip_ctx* ctx, const arma_mt_params* hparams,
 			<<<1, 256, 0, ctx->stream>>>(dparams, dseeds, geo);
 	} else if (small_blocks && wave_kernel_fits(hparams, nstreams)) {
 		noise_wave_kernel<OutT, SEM, Geo>
-			<<<div_ceil(nstreams, 4), 256, 0, ctx->stream>>>(dparams, dseeds, geo, nstreams);
+			<<<div_ceil(nstreams, 16), kPcThreads, 0, ctx->stream>>>(dparams, dseeds, geo,
+			                                                       nstreams);
 	} else if (small_blocks && seg_kernel_fits(hparams, nstreams)) {
 		noise_seg_kernel<OutT, SEM, Geo>
 			<<<div_ceil(nstreams, 2), 64, 0, ctx->stream>>>(dparams, dseeds, geo, nstreams);
