@@ -134,11 +134,15 @@ __device__ __forceinline__ void products_row4(const double* __restrict__ A,
 // factors of A skipped, so the tiling changes no bit; it cuts the L2 reads
 // of one workgroup by the tile's reuse.
 constexpr int kGT = 64, kGK = 32;
-constexpr int kPoolDoubles = kGT * (kGK + 1) + kGK * (kGT + 1);
+constexpr int kLuLdsN = 127;  // LU factorisations up to this order run in LDS (shell(6))
+constexpr int kPoolDoubles = kLuLdsN * kLuLdsN > kGT * (kGK + 1) + kGK * (kGT + 1)
+                                 ? kLuLdsN * kLuLdsN
+                                 : kGT * (kGK + 1) + kGK * (kGT + 1);
 
-// The workgroup's one LDS buffer, shared by the product tiles and the
-// substitution's row blocks (a function-scope __shared__ array is a single
-// allocation however many callers inline it).
+// The workgroup's one LDS buffer, shared by the product tiles, the LU
+// factorisations that fit and the substitution's row blocks (a
+// function-scope __shared__ array is a single allocation however many
+// callers inline it).
 __device__ __forceinline__ double* lds_pool() {
 	__shared__ double pool[kPoolDoubles];
 	return pool;
@@ -362,14 +366,10 @@ __device__ __forceinline__ void substitute(const YwArgs& y, DMat inv) {
 	__syncthreads();
 }
 
-// inv = src^{-1} by LU with partial pivoting and one substitution per column
-// (getrf + getri in the order of the host solver).  Returns false when a
-// pivot is exactly zero.
-__device__ __forceinline__ bool inverse(const YwArgs& y, DMat inv, DMat src) {
-	const int n = src.r;
-	double* lu = y.lu;
-	for (int e = threadIdx.x; e < n * n; e += blockDim.x) lu[e] = src.p[e];
-	__syncthreads();
+// LU factorisation with partial pivoting of the n x n matrix at lu (in LDS
+// or device memory; inlined per call site, so each copy knows which), the
+// pivots to y.piv.  Returns false when a pivot is exactly zero.
+__device__ __forceinline__ bool lu_factor(const YwArgs& y, double* lu, int n) {
 	__shared__ int pk;
 	__shared__ int bad;
 	__shared__ double wbest[kYwThreads / 64];
@@ -454,6 +454,29 @@ __device__ __forceinline__ bool inverse(const YwArgs& y, DMat inv, DMat src) {
 				if (r < cnt) row[j0 + r] = a[r] - l * u[r];
 		}
 		__syncthreads();
+	}
+	return true;
+}
+
+// inv = src^{-1} by LU with partial pivoting and one substitution per column
+// (getrf + getri in the order of the host solver).  Returns false when a
+// pivot is exactly zero.  A matrix that fits the LDS pool is factorised
+// there (each of its n column steps is four barriers apart, so LDS instead
+// of L2 round trips is most of the gain) and copied to y.lu for the
+// substitution; the arithmetic is the same either way.
+__device__ __forceinline__ bool inverse(const YwArgs& y, DMat inv, DMat src) {
+	const int n = src.r;
+	if (n <= kLuLdsN) {
+		double* lu = lds_pool();
+		for (int e = threadIdx.x; e < n * n; e += blockDim.x) lu[e] = src.p[e];
+		__syncthreads();
+		if (!lu_factor(y, lu, n)) return false;
+		for (int e = threadIdx.x; e < n * n; e += blockDim.x) y.lu[e] = lu[e];
+		__syncthreads();
+	} else {
+		for (int e = threadIdx.x; e < n * n; e += blockDim.x) y.lu[e] = src.p[e];
+		__syncthreads();
+		if (!lu_factor(y, y.lu, n)) return false;
 	}
 	substitute(y, inv);
 	return true;
