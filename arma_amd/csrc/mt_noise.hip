@@ -640,7 +640,9 @@ __global__ __launch_bounds__(kPcThreads) void noise_wave_kernel(
 	constexpr int A = kRoundWords / (16 * WPA);            // attempts per lane per round
 	static_assert(A * 16 * WPA == kRoundWords, "a round is whole attempts");
 	static_assert(3 * kRoundWords + kWaveNN <= kRingWords, "ring too small");
-	__shared__ __align__(16) uint32_t ring_all[4][4][kRingWords];  // [group][row][word]
+	// rows 16 words (16 banks) apart, so the four rows' stores of a twist and
+	// reads of a round fall on different banks
+	__shared__ __align__(16) uint32_t ring_all[4][4][kRingWords + 16];  // [group][row][word]
 	__shared__ int gdone[2][4];
 	__shared__ double ltab[kLogDdRows * 3];
 	stage_log_table(ltab);
