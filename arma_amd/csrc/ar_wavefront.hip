@@ -221,7 +221,7 @@ inline int mbox_width_rt(int TX, int HX, int HY) { return (kThreads / TX) * HX +
 
 // The march of one tile over all of t (P0 accumulators; a surface of lower
 // order in t has zero coefficients past its own order).
-template <class T, int TX, int P0, bool KAHAN>
+template <class T, int TX, int P0, bool KAHAN, bool BUF = false>
 __device__ __forceinline__ void wave_tile(
 	T* __restrict__ zeta, int nt, int nx, int ny, const T* __restrict__ phi0,
 	const T* __restrict__ phic, const int* __restrict__ toff, int P1, int P2, int nY,
@@ -327,6 +327,24 @@ __device__ __forceinline__ void wave_tile(
 	const int my_m = mbox_column<TX>(lx, ly, HX, HY);
 	T* const my_mbox = my_m >= 0 ? mbox + size_t(b) * total_u * NB + my_m : my_sink;
 	const size_t mstep = my_m >= 0 ? size_t(NB) : 0;
+	// BUF: the mailbox traffic goes through one buffer descriptor spanning the
+	// mailboxes from the corner neighbour's to this tile's (the host chooses
+	// it when that span fits 31 bits), as in quad_tile: a lane with no value
+	// to store or load passes the offset just past the end, which the
+	// hardware drops without a memory access.  Without it such lanes store
+	// to their sink slot and load the zero word, device-coherently, every
+	// step.
+	const int bb0 = max(b - nY - 1, 0);
+	const size_t tile_mb = size_t(total_u) * NB;  // elements of one tile's mailbox
+	const unsigned mb_end = unsigned((size_t(b - bb0) + 1) * tile_mb * sizeof(T));
+	const __amdgpu_buffer_rsrc_t mrsrc = __builtin_amdgcn_make_buffer_rsrc(
+		mbox + size_t(bb0) * tile_mb, 0, int(mb_end), kBufferDword3);
+	const unsigned moff0 =
+		my_m >= 0 ? unsigned((size_t(b - bb0) * tile_mb + my_m) * sizeof(T)) : mb_end;
+	const unsigned mstep_b = my_m >= 0 ? unsigned(NB * sizeof(T)) : 0u;
+	auto halo_off = [&](int r, int u2) {
+		return unsigned(((size_t(hnb[r] - bb0) * total_u + u2 + hdu[r]) * NB + hm[r]) * sizeof(T));
+	};
 	auto issue_loads = [&](int u2, T& e, T& h0, T& h1) {
 		const int s2 = u2 + s0;
 		const bool in = u2 < total_u;
@@ -337,10 +355,15 @@ __device__ __forceinline__ void wave_tile(
 		const int t1 = s2 - hsk[1];
 		const bool v1 = in && hin[1] && t1 >= 0 && t1 < nt;
 		e = *(ve ? zeta + size_t(t) * sxy + own_off : zero);
-		h0 = coherent_load(v0 ? mbox + (size_t(hnb[0]) * total_u + u2 + hdu[0]) * NB + hm[0]
-		                      : zero);
-		h1 = coherent_load(v1 ? mbox + (size_t(hnb[1]) * total_u + u2 + hdu[1]) * NB + hm[1]
-		                      : zero);
+		if constexpr (BUF) {
+			h0 = buffer_load_coherent<T>(mrsrc, v0 ? halo_off(0, u2) : mb_end);
+			h1 = buffer_load_coherent<T>(mrsrc, v1 ? halo_off(1, u2) : mb_end);
+		} else {
+			h0 = coherent_load(v0 ? mbox + (size_t(hnb[0]) * total_u + u2 + hdu[0]) * NB + hm[0]
+			                      : zero);
+			h1 = coherent_load(v1 ? mbox + (size_t(hnb[1]) * total_u + u2 + hdu[1]) * NB + hm[1]
+			                      : zero);
+		}
 	};
 
 	// neighbour progress as last observed by lane 0, and the flag loads in
@@ -409,7 +432,10 @@ __device__ __forceinline__ void wave_tile(
 			const T z = t < t0 ? eps : zc;
 			if (live) ring[rs_own * slot + own_lds] = z;
 			*(live ? zeta + size_t(t) * sxy + own_off : my_sink) = z;
-			coherent_store(my_mbox + size_t(u) * mstep, z);
+			if constexpr (BUF)
+				buffer_store_coherent(mrsrc, moff0 + unsigned(u) * mstep_b, z);
+			else
+				coherent_store(my_mbox + size_t(u) * mstep, z);
 #pragma unroll
 			for (int k = 1; k < P0; ++k) acc[k].add(fma(phi0[k], z, part[k]));
 #pragma unroll
@@ -772,7 +798,7 @@ __device__ __forceinline__ void quad_tile(
 // Kernel arguments stay plain restrict-qualified pointers: the coefficient
 // reads are then provably unclobbered by the kernel's stores and compile to
 // scalar loads (a struct argument costs 30-60 VGPRs per lane).
-template <class T, int TX, int P0, bool KAHAN>
+template <class T, int TX, int P0, bool KAHAN, bool BUF>
 __global__ __launch_bounds__(kThreads) void ar_wave_kernel(
 	T* __restrict__ zeta, int nt, int nx, int ny, const T* __restrict__ phi0,
 	const T* __restrict__ phic, const int* __restrict__ toff, int P1, int P2, int nY,
@@ -781,9 +807,9 @@ __global__ __launch_bounds__(kThreads) void ar_wave_kernel(
 	extern __shared__ __align__(16) unsigned char smem_raw[];
 	// unbounded waits: tiles wait only on tiles with smaller workgroup ids,
 	// which were dispatched before them
-	wave_tile<T, TX, P0, KAHAN>(zeta, nt, nx, ny, phi0, phic, toff, P1, P2, nY, progress, zero, sink,
-	                            mbox, NB, total_steps, 0, blockIdx.x, nullptr, 0, check,
-	                            smem_raw);
+	wave_tile<T, TX, P0, KAHAN, BUF>(zeta, nt, nx, ny, phi0, phic, toff, P1, P2, nY, progress, zero,
+	                                 sink, mbox, NB, total_steps, 0, blockIdx.x, nullptr, 0, check,
+	                                 smem_raw);
 }
 
 template <class T, int P0, int NT, bool KAHAN>
@@ -1178,15 +1204,25 @@ void launch_typed(arma_hip_ctx* ctx, const arma_ar_desc* d, T* zeta) {
 	ARMA_HIP_REQUIRE(lds <= 160 * 1024, ARMA_HIP_EINVAL, "AR order too large for the LDS ring");
 	KernelTimer timer(ctx, "ar_wavefront");
 	// rings above 64 KiB (P1, P2 >= 9 in FP64) use the larger LDS of gfx950
-#define ARMA_WAVE(P)                                                                          \
-	case P:                                                                                   \
+	// mailbox accesses through a buffer descriptor when the span from a
+	// tile's corner neighbour's mailbox to the end of its own fits 31 bits
+	const bool buf = (size_t(nY) + 2) * total_u * NB * sizeof(T) < (size_t(1) << 31) - 64;
+#define ARMA_WAVE_LAUNCH(P, B)                                                                \
+	{                                                                                         \
 		if (lds > 65536)                                                                      \
 			ARMA_HIP_CHECK(hipFuncSetAttribute(                                               \
-				reinterpret_cast<const void*>(&ar_wave_kernel<T, kTX, P, KAHAN>),             \
+				reinterpret_cast<const void*>(&ar_wave_kernel<T, kTX, P, KAHAN, B>),          \
 				hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));                       \
-		ar_wave_kernel<T, kTX, P, KAHAN><<<ntiles, kThreads, lds, ctx->stream>>>(             \
+		ar_wave_kernel<T, kTX, P, KAHAN, B><<<ntiles, kThreads, lds, ctx->stream>>>(          \
 			zeta, nt, nx, ny, dphi, dphi + n0, doffs, P1, P2, nY, prog, zero, sink, mbox, NB, \
 			total_steps, hc.word);                                                            \
+	}
+#define ARMA_WAVE(P)                                                                          \
+	case P:                                                                                   \
+		if (buf)                                                                              \
+			ARMA_WAVE_LAUNCH(P, true)                                                         \
+		else                                                                                  \
+			ARMA_WAVE_LAUNCH(P, false)                                                        \
 		break;
 	switch (P0) {
 		ARMA_WAVE(1)
@@ -1203,6 +1239,7 @@ void launch_typed(arma_hip_ctx* ctx, const arma_ar_desc* d, T* zeta) {
 		throw Error(ARMA_HIP_EINVAL, "bad AR order");
 	}
 #undef ARMA_WAVE
+#undef ARMA_WAVE_LAUNCH
 	ARMA_HIP_CHECK(hipGetLastError());
 	hc.verify(ctx);
 }
