@@ -878,9 +878,17 @@ bool quad_eligible(const arma_ar_desc* d) {
 	const int P0 = d->phi_shape[0], P1 = d->phi_shape[1], P2 = d->phi_shape[2];
 	const int b = batch_bucket(P0);
 	const size_t es = d->precision == ARMA_FP32 ? 4 : 8;
-	// FP64 stays on the one-lane tile: on the 1024x256x256 surface the quad
-	// tile ran 3.0 ms against 2.66 ms (its coefficients take the whole
-	// register budget at four waves per SIMD)
+	// The one-lane tile is the default in both precisions: with its mailbox
+	// traffic behind a buffer descriptor it ran 2.38 ms against the quad
+	// tile's 2.68 ms on 1024x256x256 FP32 and 16.7 against 17.2 ms on
+	// 2048x512x512 FP32 (whose 262 144 columns fit at one lane each, four
+	// workgroups per CU, but at four lanes each only a quarter at a time),
+	// and 2.44 against 3.0 ms on 1024x256x256 FP64.  ARMA_HIP_QUAD_TILE=1
+	// selects the quad tile for FP32 surfaces, single and batched alike (a
+	// point's sum is added in another order there, so the choice must be the
+	// same for a batch member and the single surface it is compared with).
+	const char* e = std::getenv("ARMA_HIP_QUAD_TILE");
+	if (!(e && e[0] == '1' && e[1] == 0)) return false;
 	if (d->precision != ARMA_FP32) return false;
 	if (b < 0 || P1 > kMaxP12 || P2 > kMaxP12 ||
 	    !quad_fits(int(es), kBatchP0[b], quad_taps(P1 * P2)))

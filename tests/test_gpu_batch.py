@@ -99,6 +99,27 @@ def test_batch_matches_single_surfaces(hip_ctx, members, prec):
     b.close()
 
 
+def test_batch_matches_single_surfaces_quad_tile(hip_ctx, members, monkeypatch):
+    """ARMA_HIP_QUAD_TILE=1 puts FP32 surfaces, single and batched, on the
+    four-lanes-per-column tile: members still equal their single surfaces,
+    and both stay within the FP32 bar of the one-lane tile's result."""
+    prec = arma_amd.FP32
+    ref, _ = singles(hip_ctx, members, prec)
+    monkeypatch.setenv("ARMA_HIP_QUAD_TILE", "1")
+    zs, fs = singles(hip_ctx, members, prec)
+    b = hip_ctx.batch(descs(hip_ctx, members, prec), [m["gamma0"] for m in members], prec)
+    arena = torch.full((b.arena_elements,), float("nan"), dtype=torch.float32, device=DEV)
+    torch.cuda.synchronize()
+    print(f"batch quad tile, {len(members)} members", flush=True)
+    b.generate(arena)
+    assert_batch_equal(b, arena, zs, fs)
+    b.close()
+    for z, r in zip(zs, ref):
+        scale = float(np.max(np.abs(r)))
+        err = np.abs(z.astype(np.float64) - r.astype(np.float64)) / scale
+        assert float(np.sqrt(np.mean(err ** 2))) <= 1e-6 and float(np.max(err)) <= 2e-5
+
+
 def test_batch_member_matches_oracle(hip_ctx, members):
     """Member 1 (64x20x33, order 5) against the FP64 oracle directly."""
     m = members[1]
