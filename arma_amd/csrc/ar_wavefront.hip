@@ -942,7 +942,7 @@ int append_quad_tables(const double* phi, int P0, int P1, int P2, int P0S, int W
 // The loop control is uniform (no atomic fetch under a thread-0 branch),
 // which keeps the compiled loop a plain scalar loop around the tile's
 // barriers.
-template <class T, int TX, bool KAHAN>
+template <class T, int TX, bool KAHAN, bool BUF = false>
 __global__ __launch_bounds__(kThreads) void ar_batch_kernel(
 	T* __restrict__ arena, const T* __restrict__ coef, const int* __restrict__ taps,
 	int* __restrict__ progress, const T* __restrict__ zero, T* __restrict__ sink,
@@ -962,9 +962,10 @@ __global__ __launch_bounds__(kThreads) void ar_batch_kernel(
 		switch (sf.bucket) {
 #define ARMA_BATCH_BODY(I)                                                                    \
 	case I:                                                                                   \
-		wave_tile<T, TX, kBatchP0[I], KAHAN>(z, sf.nt, sf.nx, sf.ny, c0, cc, tp, sf.P1, sf.P2,  \
-		                                     sf.nY, pr, zero, sink, mb, sf.NB, sf.total_steps, \
-		                                     sf.t0, b, abortw, limit, nullptr, smem_raw);      \
+		wave_tile<T, TX, kBatchP0[I], KAHAN, BUF>(z, sf.nt, sf.nx, sf.ny, c0, cc, tp, sf.P1,    \
+		                                          sf.P2, sf.nY, pr, zero, sink, mb, sf.NB,      \
+		                                          sf.total_steps, sf.t0, b, abortw, limit,      \
+		                                          nullptr, smem_raw);                           \
 		break;
 			ARMA_BATCH_BODY(0)
 			ARMA_BATCH_BODY(1)
@@ -1302,6 +1303,7 @@ constexpr double kBatchWaitLimitS = 10.0;
 // its members need; the parts run one after the other.
 struct ArBatchPart {
 	bool quad = false;
+	bool buf = true;  // every member's mailbox span fits a buffer descriptor
 	int nqueue = 0, grid = 0, max_slots = 0;
 	size_t lds = 0;
 	size_t nclear = 0;  // ints cleared before a launch: the abort word line, progress, zero word
@@ -1336,6 +1338,7 @@ void build_part(arma_hip_ctx* ctx, const ArBatch* ab, ArBatchPart* pt,
 	long long tiles = 0, mbox_elems = 0;
 	int max_tiles = 0;
 	size_t lds = 0;
+	pt->buf = true;
 	for (size_t i = 0; i < idx.size(); ++i) {
 		const arma_ar_desc* d = all[idx[i]].d;
 		const int nt = d->shape[0], nx = d->shape[1], ny = d->shape[2];
@@ -1367,6 +1370,8 @@ void build_part(arma_hip_ctx* ctx, const ArBatch* ab, ArBatchPart* pt,
 		sf.pad = 0;
 		sf.NB = std::max(1, mbox_width_rt(TX, P1 - 1, P2 - 1));
 		const long long total_u = sf.total_steps + (P1 - 1) + (P2 - 1);
+		pt->buf = pt->buf && (size_t(sf.nY) + 2) * size_t(total_u) * sf.NB * sizeof(T) <
+		                         (size_t(1) << 31) - 64;
 		sf.moff = mbox_elems;
 		mbox_elems += (long long)ntiles[i] * total_u * sf.NB;
 		mbox_elems = (mbox_elems + 63) / 64 * 64;
@@ -1472,7 +1477,11 @@ void batch_go(arma_hip_ctx* ctx, const ArBatch* ab, const ArBatchPart* pt, T* ar
 			return;
 		}
 	}
-	ar_batch_kernel<T, TX, KAHAN><<<pt->grid, kThreads, pt->lds, ctx->stream>>>(
+	// mailboxes behind a buffer descriptor when every member's span fits (its
+	// registers are fewer, so the residency computed for the other form holds)
+	auto kernel =
+		pt->buf ? &ar_batch_kernel<T, TX, KAHAN, true> : &ar_batch_kernel<T, TX, KAHAN, false>;
+	kernel<<<pt->grid, kThreads, pt->lds, ctx->stream>>>(
 		arena, (const T*)pt->coef.ptr, (const int*)pt->taps.ptr, pt->progress,
 		(const T*)pt->zero, (T*)pt->sink, (T*)pt->mbox.ptr, (const BatchSurface*)pt->surf.ptr,
 		(const int2*)pt->queue.ptr, pt->nqueue, pt->abortw, ab->limit);
