@@ -221,7 +221,7 @@ inline int mbox_width_rt(int TX, int HX, int HY) { return (kThreads / TX) * HX +
 
 // The march of one tile over all of t (P0 accumulators; a surface of lower
 // order in t has zero coefficients past its own order).
-template <class T, int TX, int P0, bool KAHAN, bool BUF = false>
+template <class T, int TX, int P0, bool KAHAN, bool BUF = false, int KD = kD>
 __device__ __forceinline__ void wave_tile(
 	T* __restrict__ zeta, int nt, int nx, int ny, const T* __restrict__ phi0,
 	const T* __restrict__ phic, const int* __restrict__ toff, int P1, int P2, int nY,
@@ -367,18 +367,18 @@ __device__ __forceinline__ void wave_tile(
 	};
 
 	// neighbour progress as last observed by lane 0, and the flag loads in
-	// flight (read kD steps after they are issued)
+	// flight (read KD steps after they are issued)
 	int known_t = total_u, known_l = total_u;
 	if (tid == 0) {
-		known_t = nb_top >= 0 ? wait_progress(top_prog, need_top(kD - 1), abortw, limit) : total_u;
+		known_t = nb_top >= 0 ? wait_progress(top_prog, need_top(KD - 1), abortw, limit) : total_u;
 		known_l =
-			nb_left >= 0 ? wait_progress(left_prog, need_left(kD - 1), abortw, limit) : total_u;
+			nb_left >= 0 ? wait_progress(left_prog, need_left(KD - 1), abortw, limit) : total_u;
 	}
 	__syncthreads();
-	T qe[kD], qh0[kD], qh1[kD];
-	int qt[kD], ql[kD];
+	T qe[KD], qh0[KD], qh1[KD];
+	int qt[KD], ql[KD];
 #pragma unroll
-	for (int d = 0; d < kD; ++d) {
+	for (int d = 0; d < KD; ++d) {
 		issue_loads(d, qe[d], qh0[d], qh1[d]);
 		qt[d] = known_t;
 		ql[d] = known_l;
@@ -445,8 +445,8 @@ __device__ __forceinline__ void wave_tile(
 		rs_own = wrap_inc(rs_own, R);
 		rs_h0 = wrap_inc(rs_h0, R);
 		rs_h1 = wrap_inc(rs_h1, R);
-		// 3. lane 0: may the loads of step u + kD go out?
-		const int u2 = u + kD;
+		// 3. lane 0: may the loads of step u + KD go out?
+		const int u2 = u + KD;
 		if (tid == 0) {
 			known_t = max(known_t, t_q);
 			known_l = max(known_l, l_q);
@@ -460,22 +460,22 @@ __device__ __forceinline__ void wave_tile(
 			l_q = nb_left >= 0 ? coherent_load(left_prog) : total_u;
 		}
 		__syncthreads();
-		// 4. publish: the stores of step u - kD were issued before the queue
+		// 4. publish: the stores of step u - KD were issued before the queue
 		//    loads consumed at this step, so every wave has seen them complete
-		if (tid == 0 && u >= kD) coherent_store(my_prog, u - kD + 1);
-		// 5. loads for step u + kD into the queue slot just consumed
+		if (tid == 0 && u >= KD) coherent_store(my_prog, u - KD + 1);
+		// 5. loads for step u + KD into the queue slot just consumed
 		issue_loads(u2, e_q, h0_q, h1_q);
 	};
 
 	int u = 0;
-	for (; u + kD <= total_u; u += kD) {
+	for (; u + KD <= total_u; u += KD) {
 #pragma unroll
-		for (int d = 0; d < kD; ++d) step(u + d, qe[d], qh0[d], qh1[d], qt[d], ql[d]);
+		for (int d = 0; d < KD; ++d) step(u + d, qe[d], qh0[d], qh1[d], qt[d], ql[d]);
 	}
-	// the last steps (fewer than kD), each on its own queue slot; the
+	// the last steps (fewer than KD), each on its own queue slot; the
 	// condition is the same on every lane
 #pragma unroll
-	for (int d = 0; d < kD; ++d)
+	for (int d = 0; d < KD; ++d)
 		if (u + d < total_u) step(u + d, qe[d], qh0[d], qh1[d], qt[d], ql[d]);
 	// every store complete, then the final count
 	__builtin_amdgcn_s_waitcnt(0);
@@ -798,7 +798,7 @@ __device__ __forceinline__ void quad_tile(
 // Kernel arguments stay plain restrict-qualified pointers: the coefficient
 // reads are then provably unclobbered by the kernel's stores and compile to
 // scalar loads (a struct argument costs 30-60 VGPRs per lane).
-template <class T, int TX, int P0, bool KAHAN, bool BUF>
+template <class T, int TX, int P0, bool KAHAN, bool BUF, int KD>
 __global__ __launch_bounds__(kThreads) void ar_wave_kernel(
 	T* __restrict__ zeta, int nt, int nx, int ny, const T* __restrict__ phi0,
 	const T* __restrict__ phic, const int* __restrict__ toff, int P1, int P2, int nY,
@@ -807,9 +807,9 @@ __global__ __launch_bounds__(kThreads) void ar_wave_kernel(
 	extern __shared__ __align__(16) unsigned char smem_raw[];
 	// unbounded waits: tiles wait only on tiles with smaller workgroup ids,
 	// which were dispatched before them
-	wave_tile<T, TX, P0, KAHAN, BUF>(zeta, nt, nx, ny, phi0, phic, toff, P1, P2, nY, progress, zero,
-	                                 sink, mbox, NB, total_steps, 0, blockIdx.x, nullptr, 0, check,
-	                                 smem_raw);
+	wave_tile<T, TX, P0, KAHAN, BUF, KD>(zeta, nt, nx, ny, phi0, phic, toff, P1, P2, nY, progress,
+	                                     zero, sink, mbox, NB, total_steps, 0, blockIdx.x, nullptr, 0,
+	                                     check, smem_raw);
 }
 
 template <class T, int P0, int NT, bool KAHAN>
@@ -1216,22 +1216,28 @@ void launch_typed(arma_hip_ctx* ctx, const arma_ar_desc* d, T* zeta) {
 	// mailbox accesses through a buffer descriptor when the span from a
 	// tile's corner neighbour's mailbox to the end of its own fits 31 bits
 	const bool buf = (size_t(nY) + 2) * total_u * NB * sizeof(T) < (size_t(1) << 31) - 64;
-#define ARMA_WAVE_LAUNCH(P, B)                                                                \
+	// prefetch distance: 3 steps when every tile has a CU to itself (one
+	// wave per SIMD), 4 with several tiles per CU (1024x256x256 FP64: 2.34
+	// against 2.44 ms; 512^3, four tiles per CU: 4.92 against 4.87 ms)
+	const bool short_queue = buf && ntiles <= std::max(ctx->num_cus, 1);
+#define ARMA_WAVE_LAUNCH(P, B, Q)                                                             \
 	{                                                                                         \
 		if (lds > 65536)                                                                      \
 			ARMA_HIP_CHECK(hipFuncSetAttribute(                                               \
-				reinterpret_cast<const void*>(&ar_wave_kernel<T, kTX, P, KAHAN, B>),          \
+				reinterpret_cast<const void*>(&ar_wave_kernel<T, kTX, P, KAHAN, B, Q>),       \
 				hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));                       \
-		ar_wave_kernel<T, kTX, P, KAHAN, B><<<ntiles, kThreads, lds, ctx->stream>>>(          \
+		ar_wave_kernel<T, kTX, P, KAHAN, B, Q><<<ntiles, kThreads, lds, ctx->stream>>>(       \
 			zeta, nt, nx, ny, dphi, dphi + n0, doffs, P1, P2, nY, prog, zero, sink, mbox, NB, \
 			total_steps, hc.word);                                                            \
 	}
 #define ARMA_WAVE(P)                                                                          \
 	case P:                                                                                   \
-		if (buf)                                                                              \
-			ARMA_WAVE_LAUNCH(P, true)                                                         \
+		if (short_queue)                                                                      \
+			ARMA_WAVE_LAUNCH(P, true, 3)                                                      \
+		else if (buf)                                                                         \
+			ARMA_WAVE_LAUNCH(P, true, kD)                                                     \
 		else                                                                                  \
-			ARMA_WAVE_LAUNCH(P, false)                                                        \
+			ARMA_WAVE_LAUNCH(P, false, kD)                                                    \
 		break;
 	switch (P0) {
 		ARMA_WAVE(1)
