@@ -37,12 +37,13 @@ struct Matrix {
 	double operator()(int i, int j) const { return v[size_t(i) * cols + j]; }
 };
 
+// c(i, j) = sum over k in order of a(i, k) b(k, j), each term rounded and then
+// added (no fused multiply-add: the rounding is the reference's)
 Matrix operator*(const Matrix& a, const Matrix& b) {
 	Matrix c(a.rows, b.cols);
 	for (int i = 0; i < a.rows; ++i)
 		for (int k = 0; k < a.cols; ++k) {
 			const double aik = a(i, k);
-			if (aik == 0.0) continue;
 			const double* brow = &b.v[size_t(k) * b.cols];
 			double* crow = &c.v[size_t(i) * c.cols];
 			for (int j = 0; j < b.cols; ++j) crow[j] += aik * brow[j];

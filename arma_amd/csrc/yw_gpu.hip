@@ -7,10 +7,10 @@
 // data-dependent stop, so it runs as one workgroup that walks the whole
 // recursion with the matrices in device memory and every product, inverse
 // and update spread over its lanes.  Every operation keeps the order of the
-// host solver's arithmetic (products summed over k in order with zero
-// factors skipped, LU with the first largest pivot, substitutions column by
-// column, dot products in order) and no contraction into FMAs, so the GPU
-// result is the host result to the last bit.
+// host solver's arithmetic (products summed over k in order, LU with the
+// first largest pivot, substitutions column by column, dot products in
+// order) and no contraction into FMAs, so the GPU result is the host result
+// to the last bit.
 #include "common.hpp"
 
 #include <algorithm>
@@ -100,12 +100,10 @@ __device__ DMat build_Ra0(const YwArgs& y, double* dst, int b) {
 
 // Elements (i, j0 .. j0+3) of A B by one lane (cnt of them: fewer at the
 // end of a row): element (i, j) is row i of A times column j of B, summed
-// over k in order with zero factors of A skipped (the host's loop).  One
-// load of A's factor serves the four sums, whose B terms are contiguous;
-// the skip is a select rather than a branch and the four sums are
-// independent, so the loads of later terms go out while earlier terms are
-// summed: each sum is the host's to the bit.  d(j, s) receives the product
-// of column j.
+// over k in order (the host's loop).  One load of A's factor serves the four
+// sums, whose B terms are contiguous, and the four sums are independent, so
+// the loads of later terms go out while earlier terms are summed: each sum
+// is the host's to the bit.  d(j, s) receives the product of column j.
 template <class D>
 __device__ __forceinline__ void products_row4(const double* __restrict__ A,
                                               const double* __restrict__ B, int k, int n, int i,
@@ -118,10 +116,7 @@ __device__ __forceinline__ void products_row4(const double* __restrict__ A,
 		const double aik = ar[q];
 		const double* bq = bc + size_t(q) * n;
 #pragma unroll
-		for (int r = 0; r < 4; ++r) {
-			const double t = s[r] + aik * bq[r < cnt ? r : 0];
-			s[r] = aik == 0.0 ? s[r] : t;
-		}
+		for (int r = 0; r < 4; ++r) s[r] = s[r] + aik * bq[r < cnt ? r : 0];
 	}
 #pragma unroll
 	for (int r = 0; r < 4; ++r)
@@ -130,9 +125,8 @@ __device__ __forceinline__ void products_row4(const double* __restrict__ A,
 
 // Products with more than a few columns go through LDS: 64 x 64 tiles of the
 // result, k in chunks of 32, a 2 x 2 block of the tile per lane (1024
-// lanes).  Each element still sums its terms over k in order with the zero
-// factors of A skipped, so the tiling changes no bit; it cuts the L2 reads
-// of one workgroup by the tile's reuse.
+// lanes).  Each element still sums its terms over k in order, so the tiling
+// changes no bit; it cuts the L2 reads of one workgroup by the tile's reuse.
 constexpr int kGT = 64, kGK = 32;
 constexpr int kLuLdsN = 127;  // LU factorisations up to this order run in LDS (shell(6))
 constexpr int kPoolDoubles = kLuLdsN * kLuLdsN > kGT * (kGK + 1) + kGK * (kGT + 1)
@@ -173,12 +167,10 @@ __device__ void tiled_products(const double* __restrict__ A, const double* __res
 				for (int q = 0; q < kn; ++q) {
 					const double a0 = sa[2 * ty][q], a1 = sa[2 * ty + 1][q];
 					const double b0 = sb[q][2 * tx], b1 = sb[q][2 * tx + 1];
-					const double t00 = s[0][0] + a0 * b0, t01 = s[0][1] + a0 * b1;
-					const double t10 = s[1][0] + a1 * b0, t11 = s[1][1] + a1 * b1;
-					s[0][0] = a0 == 0.0 ? s[0][0] : t00;
-					s[0][1] = a0 == 0.0 ? s[0][1] : t01;
-					s[1][0] = a1 == 0.0 ? s[1][0] : t10;
-					s[1][1] = a1 == 0.0 ? s[1][1] : t11;
+					s[0][0] = s[0][0] + a0 * b0;
+					s[0][1] = s[0][1] + a0 * b1;
+					s[1][0] = s[1][0] + a1 * b0;
+					s[1][1] = s[1][1] + a1 * b1;
 				}
 				__syncthreads();
 			}
