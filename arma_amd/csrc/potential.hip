@@ -482,7 +482,7 @@ __global__ __launch_bounds__(1024) void factor_waves_kernel(const T* __restrict_
                                                            double dt, double* __restrict__ kout) {
 	extern __shared__ __align__(16) unsigned char fw_smem[];
 	Summary* sm = reinterpret_cast<Summary*>(fw_smem);  // after the scan
-	double* tile = reinterpret_cast<double*>(fw_smem);  // during a row scan
+	T* tile = reinterpret_cast<T*>(fw_smem);  // during a row scan, in the input's type
 	const int i = blockIdx.x;
 	const int dir = blockIdx.y;
 	const T* z = zeta + size_t(idx_t[i]) * nx * ny;
@@ -577,7 +577,7 @@ __global__ __launch_bounds__(1024) void factor_waves_kernel(const T* __restrict_
 				for (int u = 0; u < 8; ++u) {
 					const int e = b + (u0 + u) * int(blockDim.x);
 					const int row = e / TW, col = e - row * TW;
-					if (e < n1 * TW && c0 + col < n0) tile[row * (TW + 1) + col] = double(tv[u]);
+					if (e < n1 * TW && c0 + col < n0) tile[row * (TW + 1) + col] = tv[u];
 				}
 			}
 			__syncthreads();
@@ -586,7 +586,7 @@ __global__ __launch_bounds__(1024) void factor_waves_kernel(const T* __restrict_
 				// a whole tile row: its LDS reads are issued together
 				double zr[16];
 #pragma unroll
-				for (int col = 0; col < 16; ++col) zr[col] = tile[b * (TW + 1) + col];
+				for (int col = 0; col < 16; ++col) zr[col] = double(tile[b * (TW + 1) + col]);
 #pragma unroll
 				for (int col = 0; col < 16; ++col) {
 					if (c0 + col >= 2) visit(zm, zc, zr[col]);
@@ -595,7 +595,7 @@ __global__ __launch_bounds__(1024) void factor_waves_kernel(const T* __restrict_
 				}
 			} else {
 				for (int col = 0; col < cw; ++col) {
-					const double zp = tile[b * (TW + 1) + col];
+					const double zp = double(tile[b * (TW + 1) + col]);
 					const int c = c0 + col;  // position of zp along the row
 					if (c >= 2) visit(zm, zc, zp);
 					zm = zc;
@@ -1340,7 +1340,7 @@ void run_potential(arma_hip_ctx* ctx, const arma_pot_desc* d, const T* zeta, T* 
 		KernelTimer t_fw(ctx, "pot_factor_waves");
 		const int lines = std::max(p.nx, p.ny);
 		const size_t fw_lds = std::max(sizeof(Summary) * size_t(lines),
-		                               sizeof(double) * size_t(p.nx) * (fw_tile(p.nx) + 1));
+		                               sizeof(T) * size_t(p.nx) * (fw_tile(p.nx) + 1));
 		allow_lds(&factor_waves_kernel<T>, fw_lds);
 		// both directions in one launch: the block size is the larger line count and
 		// threads past a direction's line count only take part in barriers
