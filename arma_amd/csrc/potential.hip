@@ -698,6 +698,16 @@ struct WinParts {
 		}
 		return scale * (av + cv);
 	}
+	// Whether every level from j on takes the recurrence with a_j nonzero,
+	// and W_j then, for j = j0, j0 + 1, ... in order once it does: the same
+	// operations as next, without its per-level tests.
+	__device__ __forceinline__ bool plain_from(int j0) const { return !direct && jstart <= j0; }
+	__device__ __forceinline__ R step() {
+		const R v = scale * (a + c);
+		a *= rho;
+		c *= irho;
+		return v;
+	}
 };
 
 // The reference replaces every non-finite window value by zero
@@ -818,17 +828,30 @@ __device__ __forceinline__ void col_task(lds_cpx<T>* X, int ny, int M, int nz, i
 						        zseed, dz, h, span);
 					}
 					const int jz = j0 - jseed;
+					auto put = [&](int zz, T w, T wd) {
+						if (ha)
+							dst[zz * CB * N] =
+								cpx<T>{f0[e].x * w - g0[e].y * wd, f0[e].y * w + g0[e].x * wd};
+						else
+							dst[zz * CB * N] = cpx<T>{f0[e].x * w, f0[e].y * w};
+					};
 					auto levels = [&](auto checked) {
+						// the usual case, decided per wave: both windows step by
+						// the plain recurrence at every level of the group
+						if (__all(wa.plain_from(jz) && wb.plain_from(jz))) {
+							for (int zz = 0; zz < zb; ++zz) {
+								T w, wd;
+								win_pair<T, decltype(checked)::value>(wa.step(), wb.step(), w, wd);
+								put(zz, w, wd);
+							}
+							return;
+						}
 						for (int zz = 0; zz < zb; ++zz) {
 							T w, wd;
 							win_pair<T, decltype(checked)::value>(
 								wa.next(jz + zz, zseed, dz, h), wb.next(jz + zz, zseed, dz, h), w,
 								wd);
-							if (ha)
-								dst[zz * CB * N] = cpx<T>{f0[e].x * w - g0[e].y * wd,
-								                          f0[e].y * w + g0[e].x * wd};
-							else
-								dst[zz * CB * N] = cpx<T>{f0[e].x * w, f0[e].y * w};
+							put(zz, w, wd);
 						}
 					};
 					if (checked)
