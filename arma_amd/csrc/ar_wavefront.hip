@@ -221,7 +221,7 @@ inline int mbox_width_rt(int TX, int HX, int HY) { return (kThreads / TX) * HX +
 
 // The march of one tile over all of t (P0 accumulators; a surface of lower
 // order in t has zero coefficients past its own order).
-template <class T, int TX, int P0, bool KAHAN, bool BUF = false, int KD = kD>
+template <class T, int TX, int P0, bool KAHAN, bool BUF = false, int KD = kD, bool STG = false>
 __device__ __forceinline__ void wave_tile(
 	T* __restrict__ zeta, int nt, int nx, int ny, const T* __restrict__ phi0,
 	const T* __restrict__ phic, const int* __restrict__ toff, int P1, int P2, int nY,
@@ -313,6 +313,16 @@ __device__ __forceinline__ void wave_tile(
 	int rs_own = pmod(s0 - lx - ly, R);
 	int rs_h0 = pmod(s0 - hsk[0], R);
 	int rs_h1 = pmod(s0 - hsk[1], R);
+	// STG: lane (lx, ly) computes slice t at step t + lx + ly and stores it
+	// at step t + lx + kTY, from a delay line of its own in LDS after the
+	// ring (kTY + 1 slots: the slot a step reads, kTY - ly steps old, is never
+	// the one it writes), so the lanes of a tile row store one slice at the
+	// same step: a wave's surface store covers whole row segments of four
+	// slices instead of touching 64 cache lines of as many slices.
+	constexpr int kDL = kTY + 1;
+	T* const dline = ring + R * slot;
+	int dl_w = pmod(s0 - lx - ly, kDL);   // slot of the slice computed at this step
+	int dl_r = pmod(s0 - lx - kTY, kDL);  // slot of the slice stored at this step
 
 	// Every lane issues the same global loads and stores at every step
 	// (out-of-range loads read a zero, out-of-range stores go to this tile's
@@ -431,7 +441,15 @@ __device__ __forceinline__ void wave_tile(
 			const T zc = eps + (acc[0].value() + phi000 * eps);
 			const T z = t < t0 ? eps : zc;
 			if (live) ring[rs_own * slot + own_lds] = z;
-			*(live ? zeta + size_t(t) * sxy + own_off : my_sink) = z;
+			if constexpr (STG) {
+				const int ts = s - lx - kTY;
+				const bool st = active && ts >= 0 && ts < nt;
+				const T zd = dline[dl_r * kThreads + tid];
+				*(st ? zeta + size_t(ts) * sxy + own_off : my_sink) = zd;
+				dline[dl_w * kThreads + tid] = z;
+			} else {
+				*(live ? zeta + size_t(t) * sxy + own_off : my_sink) = z;
+			}
 			if constexpr (BUF)
 				buffer_store_coherent(mrsrc, moff0 + unsigned(u) * mstep_b, z);
 			else
@@ -445,6 +463,10 @@ __device__ __forceinline__ void wave_tile(
 		rs_own = wrap_inc(rs_own, R);
 		rs_h0 = wrap_inc(rs_h0, R);
 		rs_h1 = wrap_inc(rs_h1, R);
+		if constexpr (STG) {
+			dl_w = wrap_inc(dl_w, kDL);
+			dl_r = wrap_inc(dl_r, kDL);
+		}
 		// 3. lane 0: may the loads of step u + KD go out?
 		const int u2 = u + KD;
 		if (tid == 0) {
@@ -477,6 +499,12 @@ __device__ __forceinline__ void wave_tile(
 #pragma unroll
 	for (int d = 0; d < KD; ++d)
 		if (u + d < total_u) step(u + d, qe[d], qh0[d], qh1[d], qt[d], ql[d]);
+	// STG: the stores due at step total_steps (slice nt - 1 of the tile's
+	// last row; every other row's last slice was stored inside the loop)
+	if constexpr (STG) {
+		const int ts = total_steps - lx - kTY;
+		if (active && ts >= 0 && ts < nt) zeta[size_t(ts) * sxy + own_off] = dline[dl_r * kThreads + tid];
+	}
 	// every store complete, then the final count
 	__builtin_amdgcn_s_waitcnt(0);
 	__syncthreads();
@@ -798,7 +826,7 @@ __device__ __forceinline__ void quad_tile(
 // Kernel arguments stay plain restrict-qualified pointers: the coefficient
 // reads are then provably unclobbered by the kernel's stores and compile to
 // scalar loads (a struct argument costs 30-60 VGPRs per lane).
-template <class T, int TX, int P0, bool KAHAN, bool BUF, int KD>
+template <class T, int TX, int P0, bool KAHAN, bool BUF, int KD, bool STG = false>
 __global__ __launch_bounds__(kThreads) void ar_wave_kernel(
 	T* __restrict__ zeta, int nt, int nx, int ny, const T* __restrict__ phi0,
 	const T* __restrict__ phic, const int* __restrict__ toff, int P1, int P2, int nY,
@@ -807,9 +835,9 @@ __global__ __launch_bounds__(kThreads) void ar_wave_kernel(
 	extern __shared__ __align__(16) unsigned char smem_raw[];
 	// unbounded waits: tiles wait only on tiles with smaller workgroup ids,
 	// which were dispatched before them
-	wave_tile<T, TX, P0, KAHAN, BUF, KD>(zeta, nt, nx, ny, phi0, phic, toff, P1, P2, nY, progress,
-	                                     zero, sink, mbox, NB, total_steps, 0, blockIdx.x, nullptr, 0,
-	                                     check, smem_raw);
+	wave_tile<T, TX, P0, KAHAN, BUF, KD, STG>(zeta, nt, nx, ny, phi0, phic, toff, P1, P2, nY,
+	                                          progress, zero, sink, mbox, NB, total_steps, 0,
+	                                          blockIdx.x, nullptr, 0, check, smem_raw);
 }
 
 template <class T, int P0, int NT, bool KAHAN>
@@ -1209,8 +1237,9 @@ void launch_typed(arma_hip_ctx* ctx, const arma_ar_desc* d, T* zeta) {
 	HaloCheck<T> hc;
 	hc.arm(ctx, fb + prog_bytes, mbox, size_t(ntiles) * total_u * NB);
 	const int R = P1 + P2 - 1;
-	const size_t lds = size_t(R) * (kTX + P1 - 1) * (kTY + P2 - 1) * sizeof(T);
-	ARMA_HIP_REQUIRE(lds <= 160 * 1024, ARMA_HIP_EINVAL, "AR order too large for the LDS ring");
+	const size_t ring_lds = size_t(R) * (kTX + P1 - 1) * (kTY + P2 - 1) * sizeof(T);
+	ARMA_HIP_REQUIRE(ring_lds <= 160 * 1024, ARMA_HIP_EINVAL,
+	                 "AR order too large for the LDS ring");
 	KernelTimer timer(ctx, "ar_wavefront");
 	// rings above 64 KiB (P1, P2 >= 9 in FP64) use the larger LDS of gfx950
 	// mailbox accesses through a buffer descriptor when the span from a
@@ -1220,24 +1249,34 @@ void launch_typed(arma_hip_ctx* ctx, const arma_ar_desc* d, T* zeta) {
 	// wave per SIMD), 4 with several tiles per CU (1024x256x256 FP64: 2.34
 	// against 2.44 ms; 512^3, four tiles per CU: 4.92 against 4.87 ms)
 	const bool short_queue = buf && ntiles <= std::max(ctx->num_cus, 1);
-#define ARMA_WAVE_LAUNCH(P, B, Q)                                                             \
+	// surface stores through the per-lane delay line (whole row segments per
+	// wave) when its LDS leaves as many tiles resident on a CU as before
+	const int per_cu = div_ceil(ntiles, std::max(ctx->num_cus, 1));
+	const size_t dline_lds = size_t(kTY + 1) * kThreads * sizeof(T);
+	const bool stg = buf && size_t(per_cu) * (ring_lds + dline_lds) <= 160 * 1024;
+	const size_t lds = ring_lds + (stg ? dline_lds : 0);
+#define ARMA_WAVE_LAUNCH(P, B, Q, S)                                                          \
 	{                                                                                         \
 		if (lds > 65536)                                                                      \
 			ARMA_HIP_CHECK(hipFuncSetAttribute(                                               \
-				reinterpret_cast<const void*>(&ar_wave_kernel<T, kTX, P, KAHAN, B, Q>),       \
+				reinterpret_cast<const void*>(&ar_wave_kernel<T, kTX, P, KAHAN, B, Q, S>),    \
 				hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));                       \
-		ar_wave_kernel<T, kTX, P, KAHAN, B, Q><<<ntiles, kThreads, lds, ctx->stream>>>(       \
+		ar_wave_kernel<T, kTX, P, KAHAN, B, Q, S><<<ntiles, kThreads, lds, ctx->stream>>>(    \
 			zeta, nt, nx, ny, dphi, dphi + n0, doffs, P1, P2, nY, prog, zero, sink, mbox, NB, \
 			total_steps, hc.word);                                                            \
 	}
 #define ARMA_WAVE(P)                                                                          \
 	case P:                                                                                   \
-		if (short_queue)                                                                      \
-			ARMA_WAVE_LAUNCH(P, true, 3)                                                      \
+		if (short_queue && stg)                                                               \
+			ARMA_WAVE_LAUNCH(P, true, 3, true)                                                \
+		else if (short_queue)                                                                 \
+			ARMA_WAVE_LAUNCH(P, true, 3, false)                                               \
+		else if (buf && stg)                                                                  \
+			ARMA_WAVE_LAUNCH(P, true, kD, true)                                               \
 		else if (buf)                                                                         \
-			ARMA_WAVE_LAUNCH(P, true, kD)                                                     \
+			ARMA_WAVE_LAUNCH(P, true, kD, false)                                              \
 		else                                                                                  \
-			ARMA_WAVE_LAUNCH(P, false, kD)                                                    \
+			ARMA_WAVE_LAUNCH(P, false, kD, false)                                             \
 		break;
 	switch (P0) {
 		ARMA_WAVE(1)
