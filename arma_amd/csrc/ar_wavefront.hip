@@ -221,7 +221,7 @@ inline int mbox_width_rt(int TX, int HX, int HY) { return (kThreads / TX) * HX +
 
 // The march of one tile over all of t (P0 accumulators; a surface of lower
 // order in t has zero coefficients past its own order).
-template <class T, int TX, int P0, bool KAHAN, bool BUF = false, int KD = kD, bool STG = false>
+template <class T, int TX, int P0, bool KAHAN, bool BUF = false, int KD = kD, int STG = 0>
 __device__ __forceinline__ void wave_tile(
 	T* __restrict__ zeta, int nt, int nx, int ny, const T* __restrict__ phi0,
 	const T* __restrict__ phic, const int* __restrict__ toff, int P1, int P2, int nY,
@@ -313,16 +313,24 @@ __device__ __forceinline__ void wave_tile(
 	int rs_own = pmod(s0 - lx - ly, R);
 	int rs_h0 = pmod(s0 - hsk[0], R);
 	int rs_h1 = pmod(s0 - hsk[1], R);
-	// STG: lane (lx, ly) computes slice t at step t + lx + ly and stores it
-	// at step t + lx + kTY, from a delay line of its own in LDS after the
-	// ring (kTY + 1 slots: the slot a step reads, kTY - ly steps old, is never
-	// the one it writes), so the lanes of a tile row store one slice at the
-	// same step: a wave's surface store covers whole row segments of four
-	// slices instead of touching 64 cache lines of as many slices.
+	// STG > 0: the surface stores of a tile row go to one slice per step, and
+	// with STG = 2 the noise loads too.  Lane (lx, ly) computes slice t at
+	// step t + lx + ly; its value is stored at step t + lx + kTY and (STG = 2)
+	// its noise loaded for step t + lx, as for ly = 0, through a line of
+	// kTY + 1 slots of its own in LDS after the ring: slot t mod (kTY + 1)
+	// receives the noise at step t + lx, gives it up and takes the value at
+	// step t + lx + ly, and gives that up at step t + lx + kTY, before the
+	// noise of slice t + kTY + 1 arrives.  A wave's store or load then covers
+	// whole row segments of four slices instead of touching 64 cache lines of
+	// as many slices.  The line starts zeroed (the noise of slices before the
+	// first is zero).
 	constexpr int kDL = kTY + 1;
 	T* const dline = ring + R * slot;
+	int dl_d = pmod(s0 - lx, kDL);        // slot whose noise arrives at this step
 	int dl_w = pmod(s0 - lx - ly, kDL);   // slot of the slice computed at this step
 	int dl_r = pmod(s0 - lx - kTY, kDL);  // slot of the slice stored at this step
+	if constexpr (STG > 0)
+		for (int q = 0; q < kDL; ++q) dline[q * kThreads + tid] = T(0);
 
 	// Every lane issues the same global loads and stores at every step
 	// (out-of-range loads read a zero, out-of-range stores go to this tile's
@@ -358,7 +366,7 @@ __device__ __forceinline__ void wave_tile(
 	auto issue_loads = [&](int u2, T& e, T& h0, T& h1) {
 		const int s2 = u2 + s0;
 		const bool in = u2 < total_u;
-		const int t = s2 - lx - ly;
+		const int t = STG == 2 ? s2 - lx : s2 - lx - ly;  // row-aligned noise
 		const bool ve = in && active && t >= 0 && t < nt;
 		const int t0 = s2 - hsk[0];
 		const bool v0 = in && hin[0] && t0 >= 0 && t0 < nt;
@@ -437,14 +445,21 @@ __device__ __forceinline__ void wave_tile(
 			acc[0].add(part[0]);
 			// slices t < t0 are given (a realtime stream's history): they
 			// feed the sums of later slices unchanged
-			const T eps = e_q;
+			T eps = e_q;
+			T zd = T(0);
+			// the value due for storage; with STG = 2 the noise arriving
+			// (slice s - lx) into its slot, then this lane's noise (slice t)
+			if constexpr (STG > 0) zd = dline[dl_r * kThreads + tid];
+			if constexpr (STG == 2) {
+				dline[dl_d * kThreads + tid] = e_q;
+				eps = dline[dl_w * kThreads + tid];
+			}
 			const T zc = eps + (acc[0].value() + phi000 * eps);
 			const T z = t < t0 ? eps : zc;
 			if (live) ring[rs_own * slot + own_lds] = z;
-			if constexpr (STG) {
+			if constexpr (STG > 0) {
 				const int ts = s - lx - kTY;
 				const bool st = active && ts >= 0 && ts < nt;
-				const T zd = dline[dl_r * kThreads + tid];
 				*(st ? zeta + size_t(ts) * sxy + own_off : my_sink) = zd;
 				dline[dl_w * kThreads + tid] = z;
 			} else {
@@ -463,7 +478,8 @@ __device__ __forceinline__ void wave_tile(
 		rs_own = wrap_inc(rs_own, R);
 		rs_h0 = wrap_inc(rs_h0, R);
 		rs_h1 = wrap_inc(rs_h1, R);
-		if constexpr (STG) {
+		if constexpr (STG > 0) {
+			dl_d = wrap_inc(dl_d, kDL);
 			dl_w = wrap_inc(dl_w, kDL);
 			dl_r = wrap_inc(dl_r, kDL);
 		}
@@ -501,7 +517,7 @@ __device__ __forceinline__ void wave_tile(
 		if (u + d < total_u) step(u + d, qe[d], qh0[d], qh1[d], qt[d], ql[d]);
 	// STG: the stores due at step total_steps (slice nt - 1 of the tile's
 	// last row; every other row's last slice was stored inside the loop)
-	if constexpr (STG) {
+	if constexpr (STG > 0) {
 		const int ts = total_steps - lx - kTY;
 		if (active && ts >= 0 && ts < nt) zeta[size_t(ts) * sxy + own_off] = dline[dl_r * kThreads + tid];
 	}
@@ -826,7 +842,7 @@ __device__ __forceinline__ void quad_tile(
 // Kernel arguments stay plain restrict-qualified pointers: the coefficient
 // reads are then provably unclobbered by the kernel's stores and compile to
 // scalar loads (a struct argument costs 30-60 VGPRs per lane).
-template <class T, int TX, int P0, bool KAHAN, bool BUF, int KD, bool STG = false>
+template <class T, int TX, int P0, bool KAHAN, bool BUF, int KD, int STG = 0>
 __global__ __launch_bounds__(kThreads) void ar_wave_kernel(
 	T* __restrict__ zeta, int nt, int nx, int ny, const T* __restrict__ phi0,
 	const T* __restrict__ phic, const int* __restrict__ toff, int P1, int P2, int nY,
@@ -1249,8 +1265,12 @@ void launch_typed(arma_hip_ctx* ctx, const arma_ar_desc* d, T* zeta) {
 	// wave per SIMD), 4 with several tiles per CU (1024x256x256 FP64: 2.34
 	// against 2.44 ms; 512^3, four tiles per CU: 4.92 against 4.87 ms)
 	const bool short_queue = buf && ntiles <= std::max(ctx->num_cus, 1);
-	// surface stores through the per-lane delay line (whole row segments per
-	// wave) when its LDS leaves as many tiles resident on a CU as before
+	// surface stores (and, with several tiles per CU, noise loads) through
+	// the per-lane line, whole row segments per wave, when its LDS leaves as
+	// many tiles resident on a CU as before.  2048x512x512 FP32, four tiles
+	// per CU: 16.65 ms, 9.97 with the stores, 7.50 with the loads too;
+	// 1024x256x256 FP64, one tile per CU: 2.34, 2.16, 2.19 (the loads' LDS
+	// round trip then lengthens the step's chain more than it saves)
 	const int per_cu = div_ceil(ntiles, std::max(ctx->num_cus, 1));
 	const size_t dline_lds = size_t(kTY + 1) * kThreads * sizeof(T);
 	const bool stg = buf && size_t(per_cu) * (ring_lds + dline_lds) <= 160 * 1024;
@@ -1268,15 +1288,15 @@ void launch_typed(arma_hip_ctx* ctx, const arma_ar_desc* d, T* zeta) {
 #define ARMA_WAVE(P)                                                                          \
 	case P:                                                                                   \
 		if (short_queue && stg)                                                               \
-			ARMA_WAVE_LAUNCH(P, true, 3, true)                                                \
+			ARMA_WAVE_LAUNCH(P, true, 3, 1)                                                   \
 		else if (short_queue)                                                                 \
-			ARMA_WAVE_LAUNCH(P, true, 3, false)                                               \
+			ARMA_WAVE_LAUNCH(P, true, 3, 0)                                                   \
 		else if (buf && stg)                                                                  \
-			ARMA_WAVE_LAUNCH(P, true, kD, true)                                               \
+			ARMA_WAVE_LAUNCH(P, true, kD, 2)                                                  \
 		else if (buf)                                                                         \
-			ARMA_WAVE_LAUNCH(P, true, kD, false)                                              \
+			ARMA_WAVE_LAUNCH(P, true, kD, 0)                                                  \
 		else                                                                                  \
-			ARMA_WAVE_LAUNCH(P, false, kD, false)                                             \
+			ARMA_WAVE_LAUNCH(P, false, kD, 0)                                                 \
 		break;
 	switch (P0) {
 		ARMA_WAVE(1)
