@@ -986,7 +986,7 @@ int append_quad_tables(const double* phi, int P0, int P1, int P2, int P0S, int W
 // The loop control is uniform (no atomic fetch under a thread-0 branch),
 // which keeps the compiled loop a plain scalar loop around the tile's
 // barriers.
-template <class T, int TX, bool KAHAN, bool BUF = false>
+template <class T, int TX, bool KAHAN, bool BUF = false, int STG = 0>
 __global__ __launch_bounds__(kThreads) void ar_batch_kernel(
 	T* __restrict__ arena, const T* __restrict__ coef, const int* __restrict__ taps,
 	int* __restrict__ progress, const T* __restrict__ zero, T* __restrict__ sink,
@@ -1006,10 +1006,9 @@ __global__ __launch_bounds__(kThreads) void ar_batch_kernel(
 		switch (sf.bucket) {
 #define ARMA_BATCH_BODY(I)                                                                    \
 	case I:                                                                                   \
-		wave_tile<T, TX, kBatchP0[I], KAHAN, BUF>(z, sf.nt, sf.nx, sf.ny, c0, cc, tp, sf.P1,    \
-		                                          sf.P2, sf.nY, pr, zero, sink, mb, sf.NB,      \
-		                                          sf.total_steps, sf.t0, b, abortw, limit,      \
-		                                          nullptr, smem_raw);                           \
+		wave_tile<T, TX, kBatchP0[I], KAHAN, BUF, kD, STG>(                                     \
+			z, sf.nt, sf.nx, sf.ny, c0, cc, tp, sf.P1, sf.P2, sf.nY, pr, zero, sink, mb, sf.NB, \
+			sf.total_steps, sf.t0, b, abortw, limit, nullptr, smem_raw);                        \
 		break;
 			ARMA_BATCH_BODY(0)
 			ARMA_BATCH_BODY(1)
@@ -1318,24 +1317,28 @@ void launch_typed(arma_hip_ctx* ctx, const arma_ar_desc* d, T* zeta) {
 	hc.verify(ctx);
 }
 
+// The kernel a batch part launches: the quad tile, the one-lane tile with
+// the per-lane line (stg: row-aligned noise loads and surface stores, with
+// mailboxes behind a buffer descriptor), or the plain one-lane tile.
 template <class T, int TX, bool KAHAN>
 struct BatchKernel {
-	static const void* fn(bool quad) {
+	static const void* fn(bool quad, bool stg) {
 		if constexpr (sizeof(T) == 4)
 			if (quad) return reinterpret_cast<const void*>(&ar_quad_batch_kernel<T, TX, KAHAN>);
+		if (stg) return reinterpret_cast<const void*>(&ar_batch_kernel<T, TX, KAHAN, true, 2>);
 		return reinterpret_cast<const void*>(&ar_batch_kernel<T, TX, KAHAN>);
 	}
 };
 
-const void* batch_kernel_fn(int precision, int tx, bool quad) {
+const void* batch_kernel_fn(int precision, int tx, bool quad, bool stg = false) {
 	if (precision == ARMA_FP32) {
-		if (tx == 8) return BatchKernel<float, 8, true>::fn(quad);
-		if (tx == 32) return BatchKernel<float, 32, true>::fn(quad);
-		return BatchKernel<float, 16, true>::fn(quad);
+		if (tx == 8) return BatchKernel<float, 8, true>::fn(quad, stg);
+		if (tx == 32) return BatchKernel<float, 32, true>::fn(quad, stg);
+		return BatchKernel<float, 16, true>::fn(quad, stg);
 	}
-	if (tx == 8) return BatchKernel<double, 8, false>::fn(quad);
-	if (tx == 32) return BatchKernel<double, 32, false>::fn(quad);
-	return BatchKernel<double, 16, false>::fn(quad);
+	if (tx == 8) return BatchKernel<double, 8, false>::fn(quad, stg);
+	if (tx == 32) return BatchKernel<double, 32, false>::fn(quad, stg);
+	return BatchKernel<double, 16, false>::fn(quad, stg);
 }
 
 }  // namespace
@@ -1369,6 +1372,7 @@ constexpr double kBatchWaitLimitS = 10.0;
 struct ArBatchPart {
 	bool quad = false;
 	bool buf = true;  // every member's mailbox span fits a buffer descriptor
+	bool stg = false;  // one-lane tiles through the per-lane line (implies buf)
 	int nqueue = 0, grid = 0, max_slots = 0;
 	size_t lds = 0;
 	size_t nclear = 0;  // ints cleared before a launch: the abort word line, progress, zero word
@@ -1502,6 +1506,26 @@ void build_part(arma_hip_ctx* ctx, const ArBatch* ab, ArBatchPart* pt,
 		                                   int(lds)));
 	int per_cu = 0;
 	ARMA_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, lanes, lds));
+	// the per-lane line (TY + 1 slots per lane after the ring) when the
+	// kernel that uses it keeps at least as many workgroups resident
+	pt->stg = false;
+	if (!pt->quad && pt->buf) {
+		const size_t lds_s = lds + size_t(TY + 1) * kThreads * sizeof(T);
+		const void* fs = batch_kernel_fn(ab->precision, TX, false, true);
+		if (lds_s <= 160 * 1024) {
+			if (lds_s > 65536)
+				ARMA_HIP_CHECK(hipFuncSetAttribute(
+					fs, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds_s)));
+			int per_cu_s = 0;
+			ARMA_HIP_CHECK(
+				hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_s, fs, lanes, lds_s));
+			if (per_cu_s >= per_cu) {
+				pt->stg = true;
+				pt->lds = lds = lds_s;
+				per_cu = per_cu_s;
+			}
+		}
+	}
 	pt->max_slots = std::max(per_cu, 1);
 	const int slots = ab->cfg.slots_per_cu > 0 ? std::min(ab->cfg.slots_per_cu, pt->max_slots)
 	                                           : pt->max_slots;
@@ -1544,8 +1568,9 @@ void batch_go(arma_hip_ctx* ctx, const ArBatch* ab, const ArBatchPart* pt, T* ar
 	}
 	// mailboxes behind a buffer descriptor when every member's span fits (its
 	// registers are fewer, so the residency computed for the other form holds)
-	auto kernel =
-		pt->buf ? &ar_batch_kernel<T, TX, KAHAN, true> : &ar_batch_kernel<T, TX, KAHAN, false>;
+	auto kernel = pt->stg  ? &ar_batch_kernel<T, TX, KAHAN, true, 2>
+	              : pt->buf ? &ar_batch_kernel<T, TX, KAHAN, true, 0>
+	                        : &ar_batch_kernel<T, TX, KAHAN, false, 0>;
 	kernel<<<pt->grid, kThreads, pt->lds, ctx->stream>>>(
 		arena, (const T*)pt->coef.ptr, (const int*)pt->taps.ptr, pt->progress,
 		(const T*)pt->zero, (T*)pt->sink, (T*)pt->mbox.ptr, (const BatchSurface*)pt->surf.ptr,
