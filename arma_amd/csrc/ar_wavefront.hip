@@ -1270,9 +1270,15 @@ void launch_typed(arma_hip_ctx* ctx, const arma_ar_desc* d, T* zeta) {
 	// per CU: 16.65 ms, 9.97 with the stores, 7.50 with the loads too;
 	// 1024x256x256 FP64, one tile per CU: 2.34, 2.16, 2.19 (the loads' LDS
 	// round trip then lengthens the step's chain more than it saves)
+	// It is taken too where it leaves fewer (but at least two) tiles resident
+	// per CU: tiles wait only on tiles dispatched before them, and the
+	// shorter steps outweigh the later start of the last tiles (2048x512x512
+	// FP64: 19.15 -> 9.23 ms; FP32 with phi (7,7,7): 20.1 -> 13.95 ms; 512^3
+	// FP64: 4.86 -> 3.98 ms; two or three tiles per CU instead of four).
 	const int per_cu = div_ceil(ntiles, std::max(ctx->num_cus, 1));
 	const size_t dline_lds = size_t(kTY + 1) * kThreads * sizeof(T);
-	const bool stg = buf && size_t(per_cu) * (ring_lds + dline_lds) <= 160 * 1024;
+	const bool stg = buf && (size_t(per_cu) * (ring_lds + dline_lds) <= 160 * 1024 ||
+	                         (ring_lds + dline_lds) * 2 <= 160 * 1024);
 	const size_t lds = ring_lds + (stg ? dline_lds : 0);
 #define ARMA_WAVE_LAUNCH(P, B, Q, S)                                                          \
 	{                                                                                         \
