@@ -547,9 +547,10 @@ __device__ __forceinline__ void wave_tile(
 // progress protocol and halo geometry are those of wave_tile.
 // Measured on MI355X: one tile alone steps in 0.78 us against 1.13 us for
 // wave_tile (1024 x 16 x 16, phi (7,7,2), FP64), but a full grid of tiles is
-// set by the inter-tile fill and the memory traffic of 1024 lanes, and only
-// the FP32 surfaces come out ahead (2048 x 512 x 512: 16.7 against 17.7 ms;
-// FP64 1024 x 256 x 256: 3.0 against 2.66 ms), so FP32 surfaces take it.
+// set by the inter-tile fill and the memory traffic of 1024 lanes: wave_tile
+// is faster on full grids (2048 x 512 x 512 FP32: 7.5 ms with its per-lane
+// line; FP64 1024 x 256 x 256: 2.2 against 3.0 ms), so the quad tile runs
+// only when ARMA_HIP_QUAD_TILE=1 asks for it (FP32 surfaces).
 constexpr int kQuad = 4;
 constexpr int kQThreads = kThreads * kQuad;
 constexpr int kQD = 4;  // prefetch distance of the queue loads, in steps
